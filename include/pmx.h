@@ -1,0 +1,213 @@
+/*
+ * pmx.h -- C ABI of the MI355X-native PatchMatch multi-view-stereo engine.
+ *
+ * Plain pointers and sizes only.  Pitches are in bytes (as the reference's
+ * _pitchData, src/array2d_wrapper.h:17), device pointers come from
+ * pmx_malloc_pitch / pmx_malloc, host pointers are borrowed for the call.
+ * Every entry point returns PMX_OK (0) or an error code and never exits;
+ * pmx_last_error() gives the message.  Work is ordered on the context's own
+ * HIP stream; calls that return host data synchronise that stream.
+ *
+ * Each declaration cites the reference interface it replaces.
+ */
+#ifndef PMX_H
+#define PMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMX_OK 0
+#define PMX_ERR_ARG 1         /* invalid argument (reference: message + exit) */
+#define PMX_ERR_HIP 2         /* HIP runtime error (reference: CUDA_SAFE_CALL exit) */
+#define PMX_ERR_ALLOC 3       /* device allocation failed */
+#define PMX_ERR_STATE 4       /* call not valid in the current engine state */
+#define PMX_ERR_NO_DEVICE 5   /* no usable GPU */
+
+#define PMX_BLUR_NONE 0
+#define PMX_BLUR_GAUSS 1      /* normalised Gaussian taps */
+#define PMX_BLUR_BOX 2        /* reference taps: uniform 1/width (src/GaussianBlurCUDA.cu:128-129) */
+
+#define PMX_INIT_DEPTH 0      /* (1-u)(far-near)+near (src/array2d_wrapper.cu:15) */
+#define PMX_INIT_INVERSE 1    /* inverse depth uniform in [1/far, 1/near] */
+
+#define PMX_IMG_F32 0         /* float32 intensities */
+#define PMX_IMG_U8 1          /* uchar, read as u/255 (cudaReadModeNormalizedFloat) */
+
+#define PMX_MEMCPY_H2D 1
+#define PMX_MEMCPY_D2H 2
+#define PMX_MEMCPY_D2D 3
+
+#define PMX_MAX_VIEWS 64
+#define PMX_MAX_SAMPLES 64
+#define PMX_MAX_HALF_WINDOW 26   /* src/patchMatch.cu:7 MAX_WINDOW_SIZE 53 */
+
+typedef struct pmx_ctx pmx_ctx;
+typedef struct pmx_engine pmx_engine;
+
+/* Per-pixel generator state: Marsaglia xorwow, 24 bytes.  Replaces
+ * curandState (src/array2d_psng.h:8). */
+typedef struct pmx_rng_state {
+    uint32_t x, y, z, w, v, d;
+} pmx_rng_state;
+
+/* Pinhole camera: row-major K and R, camera centre C (src/image.h:13-19). */
+typedef struct pmx_camera {
+    double K[9];
+    double R[9];
+    double C[3];
+} pmx_camera;
+
+/* Constructor arguments of PatchMatch (src/patchMatch.h:71-72) plus the
+ * RNG layout fixed in src/main.cpp:115-116. */
+typedef struct pmx_params {
+    int half_window;        /* halfWindowSize, 1..22 (src/array2d_refImg.cu:72-97) */
+    int num_samples;        /* numOfSamples (S) */
+    int num_iterations;     /* numOfIterations */
+    float depth_near;       /* nearRange */
+    float depth_far;        /* farRange */
+    float spm_alpha;        /* SPMAlpha */
+    float gaussian_sigma;   /* gaussianSigma (pre-blur; unused by the reference) */
+    int blur_mode;          /* PMX_BLUR_* */
+    int init_mode;          /* PMX_INIT_* */
+    float orientation_x;    /* orientationX */
+    float orientation_z;    /* orientationZ */
+    int block_x;            /* blockDim_x of the RNG seed layout (32) */
+    int block_y;            /* blockDim_y of the RNG seed layout (16) */
+    uint32_t rng_seed;      /* project seed mixed into every pixel's state */
+} pmx_params;
+
+/* Counters and stage timings of an engine (milliseconds, from HIP events on
+ * the engine's stream when timing is enabled). */
+typedef struct pmx_stats {
+    uint64_t ncc_evals;       /* NCC evaluations of sweep kernels since the last reset */
+    uint64_t ncc_samples;     /* their in-bounds window samples */
+    uint64_t cost_all_evals;  /* NCC evaluations of the cost-all-views kernel */
+    uint64_t cost_all_samples;
+    int sweep_launches;
+    int transpose_launches;
+    double ms_sweep;          /* summed duration of sweep kernels */
+    double ms_cost_all;       /* cost-all-views kernel */
+    double ms_transpose;      /* state transposes */
+    double ms_prepare;        /* blur + box sums + RNG init + depth init */
+    double ms_sweep_max;      /* longest single sweep launch */
+} pmx_stats;
+
+/* ------------------------------------------------------------ context */
+/* Replaces setBestGPUDevice (src/utility_CUDA.cpp:73-112) and the implicit
+ * CUDA context: device < 0 picks device 0. */
+int pmx_ctx_create(int device, pmx_ctx** out);
+int pmx_ctx_destroy(pmx_ctx* ctx);
+/* Message of the last failed call on ctx (or of the calling thread when ctx
+ * is NULL).  Never NULL. */
+const char* pmx_last_error(const pmx_ctx* ctx);
+int pmx_ctx_sync(pmx_ctx* ctx);
+/* The context's hipStream_t, for callers that order their own work after
+ * the engine's. */
+void* pmx_ctx_stream(pmx_ctx* ctx);
+int pmx_device_count(int* count);
+
+/* ------------------------------------------------------------- memory */
+/* cudaMallocPitch in Array2D_wrapper (src/array2d_wrapper.h:23-24). */
+int pmx_malloc_pitch(pmx_ctx* ctx, void** ptr, size_t* pitch_bytes,
+                     size_t width_bytes, size_t height);
+int pmx_malloc(pmx_ctx* ctx, void** ptr, size_t bytes);
+int pmx_free(pmx_ctx* ctx, void* ptr);
+/* Array2D_wrapper::copyData (src/array2d_wrapper.h:41-49). */
+int pmx_memcpy2d(pmx_ctx* ctx, void* dst, size_t dpitch, const void* src,
+                 size_t spitch, size_t width_bytes, size_t height, int kind);
+int pmx_memset(pmx_ctx* ctx, void* ptr, int value, size_t bytes);
+
+/* ------------------------------------------------- reference operators */
+/* cudaTranspose::transpose2dData (src/cudaTranspose.h:52-71):
+ * out[x][y] = in[y][x] for a width x height float plane. */
+int pmx_transpose_f32(pmx_ctx* ctx, const float* in, float* out, int width,
+                      int height, int pitch_in_bytes, int pitch_out_bytes);
+/* Array2D_psng::randStateGen + setup_kernel (src/array2d_psng.cu:3-31):
+ * one state per pixel, seeded by the block-linear id of a block_x*block_y
+ * launch grid. */
+int pmx_rng_init(pmx_ctx* ctx, pmx_rng_state* states, int pitch_bytes,
+                 int width, int height, int block_x, int block_y, uint32_t seed);
+/* Array2D_wrapper<float>::randNumGen + generate_kernel_float
+ * (src/array2d_wrapper.h:137-145, src/array2d_wrapper.cu:3-20). */
+int pmx_rng_fill_depth(pmx_ctx* ctx, pmx_rng_state* states, int state_pitch_bytes,
+                       float* out, int out_pitch_bytes, int width, int height,
+                       float depth_near, float depth_far, int init_mode);
+/* Array2d_refImg::filterImage (src/array2d_refImg.cu:70-131): copies I and
+ * writes the (2hw+1)^2 window sums of I and I^2 with zero border.  Sums are
+ * accumulated in FP64 and stored as float. */
+int pmx_box_sums(pmx_ctx* ctx, const float* img, int img_pitch_bytes, int width,
+                 int height, int half_window, float* out_I, float* out_sum_I,
+                 float* out_sum_II, int out_pitch_bytes);
+/* GaussianBlurCUDA::FilterMultipleImages (src/GaussianBlurCUDA.cu:189-199):
+ * in-place separable filter of `depth` stacked width x height planes
+ * (plane k starts at row k*height), clamp-to-edge. */
+int pmx_blur(pmx_ctx* ctx, float* data, int pitch_bytes, int width, int height,
+             int depth, float sigma, int blur_mode);
+/* Image::init_relative (src/image.cpp:4-13) for num_src views, composed in
+ * FP64 on the GPU.  H_out: host, num_src * 18 doubles (H1 row-major, H2). */
+int pmx_compose_homographies(pmx_ctx* ctx, const pmx_camera* ref,
+                             const pmx_camera* srcs, int num_src, float orientation_x,
+                             float orientation_z, double* H_out);
+
+/* ------------------------------------------------------------- engine */
+/* Argument checks of the PatchMatch constructor and runPatchMatch
+ * (src/patchMatch.cu:137-142, :265-270; src/array2d_refImg.cu:96), without
+ * touching a device.  pmx_engine_create runs the same checks. */
+int pmx_validate_params(const pmx_params* params, int num_src);
+/* PatchMatch::PatchMatch (src/patchMatch.cu:132-206).  ref_img and
+ * src_imgs are host images (image_format PMX_IMG_F32 or PMX_IMG_U8) with
+ * row pitches in bytes; sources may differ in size (packed like copyData,
+ * src/patchMatch.cu:51-130).  The images stay resident on the device. */
+int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format,
+                      int ref_width, int ref_height, const void* ref_img,
+                      int ref_pitch_bytes, int num_src, const int* src_widths,
+                      const int* src_heights, const void* const* src_imgs,
+                      const int* src_pitch_bytes, const pmx_camera* ref_cam,
+                      const pmx_camera* src_cams, pmx_engine** out);
+int pmx_engine_destroy(pmx_engine* eng);
+/* Constructor-time device work, re-runnable from the resident images:
+ * pre-blur, reference box sums, RNG init, initial depth (src/patchMatch.cu:
+ * 160-179). */
+int pmx_engine_prepare(pmx_engine* eng);
+/* PatchMatch::runPatchMatch / run<W> (src/patchMatch.cu:263-407). */
+int pmx_engine_run(pmx_engine* eng);
+/* Stages, for tests and tracing. */
+int pmx_engine_cost_all_views(pmx_engine* eng, int rotated);            /* :338-343 */
+int pmx_engine_sweep(pmx_engine* eng, int down, int rotated, int num_samples,
+                     int record_ids);                                    /* :344-391 */
+/* Host views of the engine state in the reference layouts (original frame):
+ * depth W x H; costs V planes of W x H (src/patchMatch.cu:162); used ids S
+ * planes of W x H (:170); rng W x H states. */
+int pmx_engine_get_depth(pmx_engine* eng, float* host, int pitch_bytes);
+int pmx_engine_set_depth(pmx_engine* eng, const float* host, int pitch_bytes);
+int pmx_engine_get_costs(pmx_engine* eng, float* host);
+int pmx_engine_set_costs(pmx_engine* eng, const float* host);
+int pmx_engine_get_rng(pmx_engine* eng, pmx_rng_state* host);
+int pmx_engine_set_rng(pmx_engine* eng, const pmx_rng_state* host);
+int pmx_engine_get_used_ids(pmx_engine* eng, uint8_t* host);
+/* Pre-blurred images the NCC reads: reference W x H, sources V planes of
+ * maxW x maxH (zero outside each source). */
+int pmx_engine_get_images(pmx_engine* eng, float* ref_host, float* src_host);
+int pmx_engine_get_ref_sums(pmx_engine* eng, float* sum_I, float* sum_II);
+int pmx_engine_get_homographies(pmx_engine* eng, double* host);
+int pmx_engine_get_dims(pmx_engine* eng, int* width, int* height, int* num_src,
+                        int* max_width, int* max_height);
+/* Device pointers of the output arrays in reference layout (valid after
+ * pmx_engine_export): depth W x H pitched, costs W x (H*V), used ids
+ * W x (H*S) uchar. */
+int pmx_engine_export(pmx_engine* eng, float** depth, int* depth_pitch,
+                      float** costs, int* cost_pitch, uint8_t** used_ids,
+                      int* ids_pitch);
+int pmx_engine_set_timing(pmx_engine* eng, int enabled);
+int pmx_engine_get_stats(pmx_engine* eng, pmx_stats* out);
+int pmx_engine_reset_stats(pmx_engine* eng);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

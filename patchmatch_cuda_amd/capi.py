@@ -1,0 +1,307 @@
+"""ctypes binding of the C ABI in include/pmx.h (libpmx.so).
+
+The product path is libpmx.so; there is no Python or CPU fallback: if the
+library is missing or no GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpmx.so")
+
+PMX_OK = 0
+ERR_NAMES = {1: "PMX_ERR_ARG", 2: "PMX_ERR_HIP", 3: "PMX_ERR_ALLOC", 4: "PMX_ERR_STATE",
+             5: "PMX_ERR_NO_DEVICE"}
+IMG_F32, IMG_U8 = 0, 1
+MEMCPY_H2D, MEMCPY_D2H, MEMCPY_D2D = 1, 2, 3
+
+RNG_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("z", "<u4"), ("w", "<u4"),
+                      ("v", "<u4"), ("d", "<u4")])
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_double * 9), ("R", ctypes.c_double * 9), ("C", ctypes.c_double * 3)]
+
+    @classmethod
+    def from_row(cls, row):
+        row = np.asarray(row, np.float64)
+        return cls((ctypes.c_double * 9)(*row[:9]), (ctypes.c_double * 9)(*row[9:18]),
+                   (ctypes.c_double * 3)(*row[18:21]))
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("half_window", ctypes.c_int), ("num_samples", ctypes.c_int),
+        ("num_iterations", ctypes.c_int), ("depth_near", ctypes.c_float),
+        ("depth_far", ctypes.c_float), ("spm_alpha", ctypes.c_float),
+        ("gaussian_sigma", ctypes.c_float), ("blur_mode", ctypes.c_int),
+        ("init_mode", ctypes.c_int), ("orientation_x", ctypes.c_float),
+        ("orientation_z", ctypes.c_float), ("block_x", ctypes.c_int),
+        ("block_y", ctypes.c_int), ("rng_seed", ctypes.c_uint32),
+    ]
+
+    @classmethod
+    def from_run_params(cls, rp):
+        return cls(rp.half_window, rp.num_samples, rp.num_iterations, rp.depth_near,
+                   rp.depth_far, rp.spm_alpha, rp.gaussian_sigma, rp.blur_mode, rp.init_mode,
+                   rp.orientation_x, rp.orientation_z, rp.block_x, rp.block_y, rp.rng_seed)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("ncc_evals", ctypes.c_uint64), ("ncc_samples", ctypes.c_uint64),
+        ("cost_all_evals", ctypes.c_uint64), ("cost_all_samples", ctypes.c_uint64),
+        ("sweep_launches", ctypes.c_int), ("transpose_launches", ctypes.c_int),
+        ("ms_sweep", ctypes.c_double), ("ms_cost_all", ctypes.c_double),
+        ("ms_transpose", ctypes.c_double), ("ms_prepare", ctypes.c_double),
+        ("ms_sweep_max", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+
+# name -> argtypes (all return int unless listed in _RESTYPES)
+SIGNATURES = {
+    "pmx_ctx_create": [_i, ctypes.POINTER(_vp)],
+    "pmx_ctx_destroy": [_vp],
+    "pmx_last_error": [_vp],
+    "pmx_ctx_sync": [_vp],
+    "pmx_ctx_stream": [_vp],
+    "pmx_device_count": [ctypes.POINTER(_i)],
+    "pmx_malloc_pitch": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_size_t),
+                         ctypes.c_size_t, ctypes.c_size_t],
+    "pmx_malloc": [_vp, ctypes.POINTER(_vp), ctypes.c_size_t],
+    "pmx_free": [_vp, _vp],
+    "pmx_memcpy2d": [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, ctypes.c_size_t,
+                     ctypes.c_size_t, _i],
+    "pmx_memset": [_vp, _vp, _i, ctypes.c_size_t],
+    "pmx_transpose_f32": [_vp, _vp, _vp, _i, _i, _i, _i],
+    "pmx_rng_init": [_vp, _vp, _i, _i, _i, _i, _i, ctypes.c_uint32],
+    "pmx_rng_fill_depth": [_vp, _vp, _i, _vp, _i, _i, _i, _f, _f, _i],
+    "pmx_box_sums": [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i],
+    "pmx_blur": [_vp, _vp, _i, _i, _i, _i, _f, _i],
+    "pmx_compose_homographies": [_vp, ctypes.POINTER(Camera), ctypes.POINTER(Camera), _i, _f, _f,
+                                 _vp],
+    "pmx_validate_params": [ctypes.POINTER(Params), _i],
+    "pmx_engine_create": [_vp, ctypes.POINTER(Params), _i, _i, _i, _vp, _i, _i,
+                          ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_vp),
+                          ctypes.POINTER(_i), ctypes.POINTER(Camera), ctypes.POINTER(Camera),
+                          ctypes.POINTER(_vp)],
+    "pmx_engine_destroy": [_vp],
+    "pmx_engine_prepare": [_vp],
+    "pmx_engine_run": [_vp],
+    "pmx_engine_cost_all_views": [_vp, _i],
+    "pmx_engine_sweep": [_vp, _i, _i, _i, _i],
+    "pmx_engine_get_depth": [_vp, _vp, _i],
+    "pmx_engine_set_depth": [_vp, _vp, _i],
+    "pmx_engine_get_costs": [_vp, _vp],
+    "pmx_engine_set_costs": [_vp, _vp],
+    "pmx_engine_get_rng": [_vp, _vp],
+    "pmx_engine_set_rng": [_vp, _vp],
+    "pmx_engine_get_used_ids": [_vp, _vp],
+    "pmx_engine_get_images": [_vp, _vp, _vp],
+    "pmx_engine_get_ref_sums": [_vp, _vp, _vp],
+    "pmx_engine_get_homographies": [_vp, _vp],
+    "pmx_engine_get_dims": [_vp] + [ctypes.POINTER(_i)] * 5,
+    "pmx_engine_export": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i), ctypes.POINTER(_vp),
+                          ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_i)],
+    "pmx_engine_set_timing": [_vp, _i],
+    "pmx_engine_get_stats": [_vp, ctypes.POINTER(Stats)],
+    "pmx_engine_reset_stats": [_vp],
+}
+_RESTYPES = {"pmx_last_error": ctypes.c_char_p, "pmx_ctx_stream": _vp}
+
+_lib = None
+
+
+class PmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: the HIP engine is not built "
+                               "(run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def last_error(ctx=None) -> str:
+    m = lib().pmx_last_error(ctx)
+    return m.decode() if m else ""
+
+
+def check(rc, ctx=None):
+    if rc != PMX_OK:
+        raise PmxError(rc, last_error(ctx))
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(lib().pmx_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class Context:
+    def __init__(self, device=0):
+        h = _vp()
+        check(lib().pmx_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pmx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def check(self, rc):
+        check(rc, self.h)
+
+    def sync(self):
+        self.check(lib().pmx_ctx_sync(self.h))
+
+    @property
+    def stream(self):
+        return lib().pmx_ctx_stream(self.h)
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Engine:
+    """One reference view against V source views, resident on the GPU."""
+
+    def __init__(self, ctx: Context, params, ref, srcs, cams, image_format=IMG_F32):
+        self.ctx = ctx
+        ref = np.ascontiguousarray(ref)
+        srcs = [np.ascontiguousarray(s) for s in srcs]
+        dt = np.float32 if image_format == IMG_F32 else np.uint8
+        ref = ref.astype(dt, copy=False)
+        srcs = [s.astype(dt, copy=False) for s in srcs]
+        self.H, self.W = ref.shape
+        self.V = len(srcs)
+        self.params = params if isinstance(params, Params) else Params.from_run_params(params)
+        esz = ref.itemsize
+        ws = (ctypes.c_int * self.V)(*[s.shape[1] for s in srcs])
+        hs = (ctypes.c_int * self.V)(*[s.shape[0] for s in srcs])
+        ps = (ctypes.c_int * self.V)(*[s.shape[1] * esz for s in srcs])
+        ptrs = (ctypes.c_void_p * self.V)(*[s.ctypes.data for s in srcs])
+        cams = np.asarray(cams, np.float64)
+        rc_cam = Camera.from_row(cams[0])
+        sc = (Camera * self.V)(*[Camera.from_row(cams[v + 1]) for v in range(self.V)])
+        h = _vp()
+        self._keep = (ref, srcs)
+        ctx.check(lib().pmx_engine_create(ctx.h, ctypes.byref(self.params), image_format, self.W,
+                                          self.H, _p(ref), self.W * esz, self.V, ws, hs, ptrs, ps,
+                                          ctypes.byref(rc_cam), sc, ctypes.byref(h)))
+        self.h = h
+        d = [ctypes.c_int() for _ in range(5)]
+        ctx.check(lib().pmx_engine_get_dims(self.h, *[ctypes.byref(x) for x in d]))
+        self.maxW, self.maxH = d[3].value, d[4].value
+        self.S = self.params.num_samples
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pmx_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _c(self, rc):
+        self.ctx.check(rc)
+
+    def prepare(self):
+        self._c(lib().pmx_engine_prepare(self.h))
+
+    def run(self):
+        self._c(lib().pmx_engine_run(self.h))
+
+    def cost_all_views(self, rotated=True):
+        self._c(lib().pmx_engine_cost_all_views(self.h, int(rotated)))
+
+    def sweep(self, down, rotated, S, record_ids=False):
+        self._c(lib().pmx_engine_sweep(self.h, int(down), int(rotated), int(S), int(record_ids)))
+
+    def depth(self):
+        out = np.zeros((self.H, self.W), np.float32)
+        self._c(lib().pmx_engine_get_depth(self.h, _p(out), self.W * 4))
+        return out
+
+    def set_depth(self, d):
+        d = np.ascontiguousarray(d, np.float32)
+        self._c(lib().pmx_engine_set_depth(self.h, _p(d), self.W * 4))
+
+    def costs(self):
+        out = np.zeros((self.V, self.H, self.W), np.float32)
+        self._c(lib().pmx_engine_get_costs(self.h, _p(out)))
+        return out
+
+    def set_costs(self, c):
+        c = np.ascontiguousarray(c, np.float32)
+        assert c.shape == (self.V, self.H, self.W)
+        self._c(lib().pmx_engine_set_costs(self.h, _p(c)))
+
+    def rng(self):
+        out = np.zeros((self.H, self.W), RNG_DTYPE)
+        self._c(lib().pmx_engine_get_rng(self.h, _p(out)))
+        return out
+
+    def set_rng(self, st):
+        st = np.ascontiguousarray(st)
+        assert st.dtype == RNG_DTYPE and st.shape == (self.H, self.W)
+        self._c(lib().pmx_engine_set_rng(self.h, _p(st)))
+
+    def used_ids(self):
+        out = np.zeros((self.S, self.H, self.W), np.uint8)
+        self._c(lib().pmx_engine_get_used_ids(self.h, _p(out)))
+        return out
+
+    def images(self):
+        ref = np.zeros((self.H, self.W), np.float32)
+        srcs = np.zeros((self.V, self.maxH, self.maxW), np.float32)
+        self._c(lib().pmx_engine_get_images(self.h, _p(ref), _p(srcs)))
+        return ref, srcs
+
+    def ref_sums(self):
+        a = np.zeros((self.H, self.W), np.float32)
+        b = np.zeros((self.H, self.W), np.float32)
+        self._c(lib().pmx_engine_get_ref_sums(self.h, _p(a), _p(b)))
+        return a, b
+
+    def homographies(self):
+        out = np.zeros((self.V, 18), np.float64)
+        self._c(lib().pmx_engine_get_homographies(self.h, _p(out)))
+        return out
+
+    def set_timing(self, on=True):
+        self._c(lib().pmx_engine_set_timing(self.h, int(on)))
+
+    def stats(self):
+        s = Stats()
+        self._c(lib().pmx_engine_get_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._c(lib().pmx_engine_reset_stats(self.h))

@@ -1,0 +1,987 @@
+/*
+ * pmx_capi.hip -- C ABI of the engine (declared in include/pmx.h).
+ *
+ * Owns the device buffers of one reference view and drives the schedule of
+ * PatchMatch::run<W> (src/patchMatch.cu:307-407): per iteration a rotated
+ * (left-to-right) sweep, a plain top-to-bottom sweep, a rotated
+ * right-to-left sweep and a plain bottom-to-top sweep, with the state
+ * transposed between the rotated and plain frames.  Unlike the reference it
+ * allocates everything once (no malloc/free per sweep, src/patchMatch.cu:
+ * 250-260, :337-391) and never synchronises between launches.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pmx_internal.h"
+
+namespace {
+
+thread_local std::string g_thread_err;
+
+int fail(pmx_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    g_thread_err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((ctx), PMX_ERR_HIP, "%s in %s at line %d", hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                               \
+    } while (0)
+
+#define PMXCHK(expr)             \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_ != PMX_OK) return rc_; \
+    } while (0)
+
+int dev_alloc(pmx_ctx* ctx, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(ctx, PMX_ERR_ALLOC, "device allocation of %zu bytes failed: %s", bytes,
+                    hipGetErrorString(e));
+    }
+    return PMX_OK;
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+int blur_taps(float sigma, int mode, float* taps) {
+    /* src/GaussianBlurCUDA.cu:99-130: width int(5 sigma), odd, in [5, 45]. */
+    int width = (int)(5.0f * sigma);
+    if (width % 2 == 0) width += 1;
+    int sz = (width - 1) >> 1;
+    if (width > 45) {
+        sz = 22;
+        width = 45;
+    } else if (width < 5) {
+        sz = 2;
+        width = 5;
+    }
+    if (mode == PMX_BLUR_BOX) {
+        for (int i = 0; i < width; i++) taps[i] = 1.0f / (float)width;
+        return width;
+    }
+    double t[45], sum = 0.0;
+    const double rv = -0.5 / ((double)sigma * (double)sigma);
+    for (int i = -sz; i <= sz; i++) {
+        t[i + sz] = std::exp((double)(i * i) * rv);
+        sum += t[i + sz];
+    }
+    for (int i = 0; i < width; i++) taps[i] = (float)(t[i] / sum);
+    return width;
+}
+
+/* ---- event timing ---- */
+struct Span {
+    int kind;
+    hipEvent_t a, b;
+};
+enum { K_SWEEP = 0, K_COST_ALL, K_TRANSPOSE, K_PREPARE };
+
+struct Timer {
+    pmx_engine* e;
+    std::vector<Span> spans;
+    hipEvent_t get() {
+        if (!e->ev_pool.empty()) {
+            hipEvent_t ev = e->ev_pool.back();
+            e->ev_pool.pop_back();
+            return ev;
+        }
+        hipEvent_t ev = nullptr;
+        if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+        return ev;
+    }
+    int begin(int kind) {
+        if (!e->timing) return -1;
+        Span s{kind, get(), get()};
+        if (!s.a || !s.b) return -1;
+        (void)hipEventRecord(s.a, e->ctx->stream);
+        spans.push_back(s);
+        return (int)spans.size() - 1;
+    }
+    void end(int id) {
+        if (id < 0) return;
+        (void)hipEventRecord(spans[id].b, e->ctx->stream);
+    }
+    void flush() {
+        if (spans.empty()) return;
+        (void)hipStreamSynchronize(e->ctx->stream);
+        for (auto& s : spans) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, s.a, s.b);
+            switch (s.kind) {
+            case K_SWEEP:
+                e->stats_acc.ms_sweep += ms;
+                if (ms > e->stats_acc.ms_sweep_max) e->stats_acc.ms_sweep_max = ms;
+                break;
+            case K_COST_ALL: e->stats_acc.ms_cost_all += ms; break;
+            case K_TRANSPOSE: e->stats_acc.ms_transpose += ms; break;
+            default: e->stats_acc.ms_prepare += ms; break;
+            }
+            e->ev_pool.push_back(s.a);
+            e->ev_pool.push_back(s.b);
+        }
+        spans.clear();
+    }
+};
+
+PmxSweepArgs sweep_args(pmx_engine* e, int rotated) {
+    PmxSweepArgs a{};
+    a.src = e->src + (size_t)PMX_SRC_PAD * e->src_pitch + PMX_SRC_PAD;
+    a.src_layer = e->src_layer;
+    a.src_pitch = e->src_pitch;
+    a.srcW = e->maxW;
+    a.srcH = e->maxH;
+    a.ref = e->ref;
+    a.ref_pitch = e->ref_pitch;
+    a.stats = e->stats;
+    a.stats_pitch = e->stats_pitch;
+    a.Hm = e->Hm;
+    if (rotated) {
+        a.depth = e->depth_rm;
+        a.depth_ld = e->depth_rm_ld;
+        a.cost = e->cost_rm;
+        a.cost_ld = e->cost_rm_ld;
+        a.Ws = e->H;
+        a.Hs = e->W;
+    } else {
+        a.depth = e->depth_cm;
+        a.depth_ld = e->depth_cm_ld;
+        a.cost = e->cost_cm;
+        a.cost_ld = e->cost_cm_ld;
+        a.Ws = e->W;
+        a.Hs = e->H;
+    }
+    a.spm = e->spm;
+    a.spm_ld = a.Hs * e->V;
+    a.rng = e->rng;
+    a.rng_pitch = e->rng_pitch;
+    a.ids = nullptr;
+    a.ids_pitch = e->ids_pitch;
+    a.counters = e->counters;
+    a.W = e->W;
+    a.H = e->H;
+    a.V = e->V;
+    a.hw = e->p.half_window;
+    a.S = 1;
+    a.rotated = rotated;
+    a.down = 1;
+    a.near_d = e->p.depth_near;
+    a.far_d = e->p.depth_far;
+    /* SPMAlphaSquare (src/patchMatch.cu:316), formed in FP64 */
+    a.alpha2 = (double)e->p.spm_alpha * (double)e->p.spm_alpha;
+    a.oX = e->p.orientation_x;
+    a.oZ = e->p.orientation_z;
+    a.iK0 = e->invK[0];
+    a.iK1 = e->invK[1];
+    return a;
+}
+
+/* Moves the state (depth + V costs) between the row-major and column-major
+ * buffers. */
+int to_layout(pmx_engine* e, int cm, Timer* tm, bool with_costs = true) {
+    if (e->layout_cm == cm) return PMX_OK;
+    pmx_ctx* ctx = e->ctx;
+    const int id = tm ? tm->begin(K_TRANSPOSE) : -1;
+    if (cm) {
+        HIPCHK(ctx, pmx_launch_transpose_rec(ctx->stream, e->depth_rm, e->depth_cm, e->W, e->H, 1,
+                                             e->depth_rm_ld, e->depth_cm_ld));
+        if (with_costs)
+            HIPCHK(ctx, pmx_launch_transpose_rec(ctx->stream, e->cost_rm, e->cost_cm, e->W, e->H,
+                                                 e->V, e->cost_rm_ld, e->cost_cm_ld));
+    } else {
+        HIPCHK(ctx, pmx_launch_transpose_rec(ctx->stream, e->depth_cm, e->depth_rm, e->H, e->W, 1,
+                                             e->depth_cm_ld, e->depth_rm_ld));
+        if (with_costs)
+            HIPCHK(ctx, pmx_launch_transpose_rec(ctx->stream, e->cost_cm, e->cost_rm, e->H, e->W,
+                                                 e->V, e->cost_cm_ld, e->cost_rm_ld));
+    }
+    if (tm) tm->end(id);
+    e->stats_acc.transpose_launches += with_costs ? 2 : 1;
+    e->layout_cm = cm;
+    return PMX_OK;
+}
+
+int do_cost_all(pmx_engine* e, int rotated, Timer* tm) {
+    PMXCHK(to_layout(e, 0, tm, false));
+    PmxSweepArgs a = sweep_args(e, 1);
+    a.rotated = rotated;
+    const int id = tm ? tm->begin(K_COST_ALL) : -1;
+    HIPCHK(e->ctx, pmx_launch_cost_all(e->ctx->stream, a));
+    if (tm) tm->end(id);
+    return PMX_OK;
+}
+
+int do_sweep(pmx_engine* e, int down, int rotated, int S, bool ids, Timer* tm) {
+    PMXCHK(to_layout(e, rotated ? 0 : 1, tm));
+    PmxSweepArgs a = sweep_args(e, rotated);
+    a.down = down;
+    a.S = S;
+    if (ids) {
+        if (S > e->ids_layers)
+            return fail(e->ctx, PMX_ERR_ARG, "num_samples %d exceeds the %d used-id planes", S,
+                        e->ids_layers);
+        a.ids = e->used_ids;
+    }
+    const int id = tm ? tm->begin(K_SWEEP) : -1;
+    HIPCHK(e->ctx, pmx_launch_sweep(e->ctx->stream, a));
+    if (tm) tm->end(id);
+    e->stats_acc.sweep_launches++;
+    return PMX_OK;
+}
+
+int check_engine(pmx_engine* e) {
+    if (!e) return fail(nullptr, PMX_ERR_ARG, "engine is NULL");
+    return PMX_OK;
+}
+
+int require_prepared(pmx_engine* e) {
+    if (!e->prepared)
+        return fail(e->ctx, PMX_ERR_STATE, "engine not prepared: call pmx_engine_prepare first");
+    return PMX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+/* ------------------------------------------------------------ context */
+
+int pmx_device_count(int* count) {
+    if (!count) return fail(nullptr, PMX_ERR_ARG, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *count = n;
+    return PMX_OK;
+}
+
+int pmx_ctx_create(int device, pmx_ctx** out) {
+    if (!out) return fail(nullptr, PMX_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return fail(nullptr, PMX_ERR_NO_DEVICE, "no HIP device is available");
+    }
+    if (device < 0) device = 0;
+    if (device >= n)
+        return fail(nullptr, PMX_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    pmx_ctx* ctx = new pmx_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, PMX_ERR_HIP, "context creation failed: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return PMX_OK;
+}
+
+int pmx_ctx_destroy(pmx_ctx* ctx) {
+    if (!ctx) return PMX_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return PMX_OK;
+}
+
+const char* pmx_last_error(const pmx_ctx* ctx) {
+    if (ctx) return ctx->err.c_str();
+    return g_thread_err.c_str();
+}
+
+int pmx_ctx_sync(pmx_ctx* ctx) {
+    if (!ctx) return fail(nullptr, PMX_ERR_ARG, "ctx is NULL");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PMX_OK;
+}
+
+void* pmx_ctx_stream(pmx_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+/* ------------------------------------------------------------- memory */
+
+int pmx_malloc_pitch(pmx_ctx* ctx, void** ptr, size_t* pitch, size_t width_bytes, size_t height) {
+    if (!ctx || !ptr || !pitch) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_malloc_pitch");
+    *ptr = nullptr;
+    hipError_t e = hipMallocPitch(ptr, pitch, width_bytes ? width_bytes : 1, height ? height : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return fail(ctx, PMX_ERR_ALLOC, "hipMallocPitch(%zu x %zu) failed: %s", width_bytes, height,
+                    hipGetErrorString(e));
+    }
+    return PMX_OK;
+}
+
+int pmx_malloc(pmx_ctx* ctx, void** ptr, size_t bytes) {
+    if (!ctx || !ptr) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_malloc");
+    return dev_alloc(ctx, ptr, bytes);
+}
+
+int pmx_free(pmx_ctx* ctx, void* ptr) {
+    if (!ctx) return fail(nullptr, PMX_ERR_ARG, "ctx is NULL");
+    if (ptr) HIPCHK(ctx, hipFree(ptr));
+    return PMX_OK;
+}
+
+int pmx_memcpy2d(pmx_ctx* ctx, void* dst, size_t dpitch, const void* src, size_t spitch,
+                 size_t width_bytes, size_t height, int kind) {
+    if (!ctx || !dst || !src) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_memcpy2d");
+    hipMemcpyKind k;
+    switch (kind) {
+    case PMX_MEMCPY_H2D: k = hipMemcpyHostToDevice; break;
+    case PMX_MEMCPY_D2H: k = hipMemcpyDeviceToHost; break;
+    case PMX_MEMCPY_D2D: k = hipMemcpyDeviceToDevice; break;
+    default: return fail(ctx, PMX_ERR_ARG, "unknown copy kind %d", kind);
+    }
+    HIPCHK(ctx, hipMemcpy2DAsync(dst, dpitch, src, spitch, width_bytes, height, k, ctx->stream));
+    if (kind != PMX_MEMCPY_D2D) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PMX_OK;
+}
+
+int pmx_memset(pmx_ctx* ctx, void* ptr, int value, size_t bytes) {
+    if (!ctx || !ptr) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_memset");
+    HIPCHK(ctx, hipMemsetAsync(ptr, value, bytes, ctx->stream));
+    return PMX_OK;
+}
+
+/* ------------------------------------------------- reference operators */
+
+int pmx_transpose_f32(pmx_ctx* ctx, const float* in, float* out, int width, int height,
+                      int pitch_in, int pitch_out) {
+    if (!ctx || !in || !out) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_transpose_f32");
+    if (width <= 0 || height <= 0)
+        return fail(ctx, PMX_ERR_ARG, "invalid transpose size %d x %d", width, height);
+    if (pitch_in % 4 || pitch_out % 4 || pitch_in < width * 4 || pitch_out < height * 4)
+        return fail(ctx, PMX_ERR_ARG, "invalid pitches %d / %d for %d x %d", pitch_in, pitch_out,
+                    width, height);
+    HIPCHK(ctx, pmx_launch_transpose_f32(ctx->stream, in, out, width, height, pitch_in / 4,
+                                         pitch_out / 4));
+    return PMX_OK;
+}
+
+int pmx_rng_init(pmx_ctx* ctx, pmx_rng_state* states, int pitch, int width, int height,
+                 int block_x, int block_y, uint32_t seed) {
+    if (!ctx || !states) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_rng_init");
+    if (width <= 0 || height <= 0 || block_x <= 0 || block_y <= 0 ||
+        pitch < width * (int)sizeof(pmx_rng_state))
+        return fail(ctx, PMX_ERR_ARG, "invalid rng grid %d x %d (block %d x %d, pitch %d)", width,
+                    height, block_x, block_y, pitch);
+    HIPCHK(ctx, pmx_launch_rng_init(ctx->stream, states, pitch, width, height, block_x, block_y,
+                                    seed));
+    return PMX_OK;
+}
+
+int pmx_rng_fill_depth(pmx_ctx* ctx, pmx_rng_state* states, int state_pitch, float* out,
+                       int out_pitch, int width, int height, float near_d, float far_d,
+                       int init_mode) {
+    if (!ctx || !states || !out) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_rng_fill_depth");
+    if (width <= 0 || height <= 0 || out_pitch % 4 || out_pitch < width * 4)
+        return fail(ctx, PMX_ERR_ARG, "invalid depth fill %d x %d pitch %d", width, height, out_pitch);
+    if (!(near_d > 0.0f) || !(far_d > near_d))
+        return fail(ctx, PMX_ERR_ARG, "invalid depth range [%g, %g]", near_d, far_d);
+    HIPCHK(ctx, pmx_launch_fill_depth(ctx->stream, states, state_pitch, out, out_pitch / 4, width,
+                                      height, near_d, far_d, init_mode));
+    return PMX_OK;
+}
+
+int pmx_box_sums(pmx_ctx* ctx, const float* img, int img_pitch, int width, int height, int hw,
+                 float* out_I, float* out_sI, float* out_sII, int out_pitch) {
+    if (!ctx || !img) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_box_sums");
+    if (hw < 1 || hw > 22)
+        return fail(ctx, PMX_ERR_ARG, "Warning: too large size of windows (half window %d, 1..22)", hw);
+    if (width <= 0 || height <= 0 || img_pitch % 4 || out_pitch % 4)
+        return fail(ctx, PMX_ERR_ARG, "invalid box-sum geometry");
+    void* tmp = nullptr;
+    PMXCHK(dev_alloc(ctx, &tmp, sizeof(double) * 2 * (size_t)width * height));
+    hipError_t e = pmx_launch_box_sums(ctx->stream, img, img_pitch / 4, width, height, hw, out_I,
+                                       out_sI, out_sII, out_pitch / 4, nullptr, 0, (double*)tmp);
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    dev_free(tmp);
+    HIPCHK(ctx, e);
+    HIPCHK(ctx, e2);
+    return PMX_OK;
+}
+
+int pmx_blur(pmx_ctx* ctx, float* data, int pitch, int width, int height, int depth, float sigma,
+             int mode) {
+    if (!ctx || !data) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_blur");
+    if (mode == PMX_BLUR_NONE) return PMX_OK;
+    if (mode != PMX_BLUR_GAUSS && mode != PMX_BLUR_BOX)
+        return fail(ctx, PMX_ERR_ARG, "unknown blur mode %d", mode);
+    if (!(sigma > 0.0f) || width <= 0 || height <= 0 || depth <= 0 || pitch % 4)
+        return fail(ctx, PMX_ERR_ARG, "invalid blur arguments");
+    float taps[45];
+    const int n = blur_taps(sigma, mode, taps);
+    void* tmp = nullptr;
+    PMXCHK(dev_alloc(ctx, &tmp, sizeof(float) * (size_t)width * height));
+    hipError_t e = pmx_launch_blur(ctx->stream, data, pitch / 4, width, height, depth,
+                                   (long)(pitch / 4) * height, taps, n, (float*)tmp);
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    dev_free(tmp);
+    HIPCHK(ctx, e);
+    HIPCHK(ctx, e2);
+    return PMX_OK;
+}
+
+int pmx_compose_homographies(pmx_ctx* ctx, const pmx_camera* ref, const pmx_camera* srcs,
+                             int num_src, float oX, float oZ, double* H_out) {
+    if (!ctx || !ref || !srcs || !H_out || num_src <= 0)
+        return fail(ctx, PMX_ERR_ARG, "invalid arguments to pmx_compose_homographies");
+    std::vector<double> cams((size_t)(num_src + 1) * 21);
+    auto pack = [&](const pmx_camera& c, double* o) {
+        std::memcpy(o, c.K, 9 * sizeof(double));
+        std::memcpy(o + 9, c.R, 9 * sizeof(double));
+        std::memcpy(o + 18, c.C, 3 * sizeof(double));
+    };
+    pack(*ref, cams.data());
+    for (int v = 0; v < num_src; v++) pack(srcs[v], cams.data() + (size_t)(v + 1) * 21);
+    void *dc = nullptr, *dh = nullptr;
+    PMXCHK(dev_alloc(ctx, &dc, cams.size() * sizeof(double)));
+    int rc = dev_alloc(ctx, &dh, (size_t)num_src * 18 * sizeof(double));
+    if (rc != PMX_OK) {
+        dev_free(dc);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(dc, cams.data(), cams.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = pmx_launch_compose(ctx->stream, (const double*)dc, num_src, oX, oZ, (double*)dh);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(H_out, dh, (size_t)num_src * 18 * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(dc);
+    dev_free(dh);
+    HIPCHK(ctx, e);
+    return PMX_OK;
+}
+
+/* ------------------------------------------------------------- engine */
+
+static void engine_free(pmx_engine* e) {
+    if (!e) return;
+    void* bufs[] = {e->src_raw, e->src,     e->ref_raw,  e->ref,     e->sum_I,    e->sum_II,
+                    e->stats,   e->Hm,      e->depth_rm, e->depth_cm, e->cost_rm, e->cost_cm,
+                    e->spm,     e->rng,     e->used_ids, e->out_depth, e->out_cost, e->counters};
+    for (void* b : bufs) dev_free(b);
+    for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+    delete e;
+}
+
+static int validate(pmx_ctx* ctx, const pmx_params* params, int num_src) {
+    if (!params) return fail(ctx, PMX_ERR_ARG, "params is NULL");
+    /* src/patchMatch.cu:137-142 */
+    if (num_src < 1) return fail(ctx, PMX_ERR_ARG, "Error: at least 2 images are needed for stereo");
+    if (num_src > PMX_MAX_VIEWS)
+        return fail(ctx, PMX_ERR_ARG, "at most %d source views are supported", PMX_MAX_VIEWS);
+    const pmx_params& p = *params;
+    /* src/patchMatch.cu:265-270 and src/array2d_refImg.cu:96 */
+    if (p.half_window < 1 || p.half_window > 22)
+        return fail(ctx, PMX_ERR_ARG,
+                    "Warning: too large size of windows (half window %d, supported 1..22)",
+                    p.half_window);
+    if (p.num_samples < 1 || p.num_samples > PMX_MAX_SAMPLES)
+        return fail(ctx, PMX_ERR_ARG, "numOfSamples %d outside 1..%d", p.num_samples, PMX_MAX_SAMPLES);
+    if (p.num_iterations < 0) return fail(ctx, PMX_ERR_ARG, "numOfIterations must be >= 0");
+    if (!(p.depth_near > 0.0f) || !(p.depth_far > p.depth_near))
+        return fail(ctx, PMX_ERR_ARG, "invalid depth range [%g, %g]", p.depth_near, p.depth_far);
+    if (!(p.spm_alpha > 0.0f)) return fail(ctx, PMX_ERR_ARG, "SPMAlpha must be positive");
+    if (p.blur_mode != PMX_BLUR_NONE && p.blur_mode != PMX_BLUR_GAUSS && p.blur_mode != PMX_BLUR_BOX)
+        return fail(ctx, PMX_ERR_ARG, "unknown blur mode %d", p.blur_mode);
+    if (p.blur_mode != PMX_BLUR_NONE && !(p.gaussian_sigma > 0.0f))
+        return fail(ctx, PMX_ERR_ARG, "gaussianSigma must be positive when blurring");
+    if (p.init_mode != PMX_INIT_DEPTH && p.init_mode != PMX_INIT_INVERSE)
+        return fail(ctx, PMX_ERR_ARG, "unknown init mode %d", p.init_mode);
+    if (p.block_x < 1 || p.block_y < 1) return fail(ctx, PMX_ERR_ARG, "invalid RNG block layout");
+    return PMX_OK;
+}
+
+int pmx_validate_params(const pmx_params* params, int num_src) {
+    return validate(nullptr, params, num_src);
+}
+
+int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format, int ref_width,
+                      int ref_height, const void* ref_img, int ref_pitch, int num_src,
+                      const int* src_widths, const int* src_heights, const void* const* src_imgs,
+                      const int* src_pitch, const pmx_camera* ref_cam, const pmx_camera* src_cams,
+                      pmx_engine** out) {
+    if (!ctx) return fail(nullptr, PMX_ERR_ARG, "ctx is NULL");
+    if (!out || !params || !ref_img || !ref_cam)
+        return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_engine_create");
+    *out = nullptr;
+    PMXCHK(validate(ctx, params, num_src));
+    if (!src_widths || !src_heights || !src_imgs || !src_pitch || !src_cams)
+        return fail(ctx, PMX_ERR_ARG, "NULL source arrays");
+    const pmx_params& p = *params;
+    if (image_format != PMX_IMG_F32 && image_format != PMX_IMG_U8)
+        return fail(ctx, PMX_ERR_ARG, "unknown image format %d", image_format);
+    const int esz = image_format == PMX_IMG_F32 ? 4 : 1;
+    if (ref_width < 1 || ref_height < 1 || ref_pitch < ref_width * esz)
+        return fail(ctx, PMX_ERR_ARG, "invalid reference image %d x %d pitch %d", ref_width,
+                    ref_height, ref_pitch);
+    int maxW = 0, maxH = 0;
+    for (int v = 0; v < num_src; v++) {
+        if (!src_imgs[v] || src_widths[v] < 1 || src_heights[v] < 1 ||
+            src_pitch[v] < src_widths[v] * esz)
+            return fail(ctx, PMX_ERR_ARG, "invalid source image %d", v);
+        if (src_widths[v] > maxW) maxW = src_widths[v];
+        if (src_heights[v] > maxH) maxH = src_heights[v];
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+
+    pmx_engine* e = new pmx_engine();
+    e->ctx = ctx;
+    e->p = p;
+    e->W = ref_width;
+    e->H = ref_height;
+    e->V = num_src;
+    e->maxW = maxW;
+    e->maxH = maxH;
+    e->image_format = image_format;
+    e->ref_cam = *ref_cam;
+    e->src_cams.assign(src_cams, src_cams + num_src);
+    e->src_w.assign(src_widths, src_widths + num_src);
+    e->src_h.assign(src_heights, src_heights + num_src);
+    const int W = e->W, H = e->H, V = e->V;
+    hipStream_t s = ctx->stream;
+    int rc = PMX_OK;
+#define ALLOC(ptr, bytes)                                                   \
+    do {                                                                    \
+        void* q_ = nullptr;                                                 \
+        rc = dev_alloc(ctx, &q_, (bytes));                                  \
+        if (rc != PMX_OK) {                                                 \
+            engine_free(e);                                                 \
+            return rc;                                                      \
+        }                                                                   \
+        ptr = (decltype(ptr))q_;                                            \
+    } while (0)
+#define HIPCHK_E(expr)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            engine_free(e);                                                               \
+            return fail(ctx, PMX_ERR_HIP, "%s in %s at line %d", hipGetErrorString(e_),   \
+                        __FILE__, __LINE__);                                              \
+        }                                                                                 \
+    } while (0)
+
+    /* sources: padded block */
+    e->src_pitch = round_up(maxW + 2 * PMX_SRC_PAD, 64);
+    e->src_layer = (long)e->src_pitch * (maxH + 2 * PMX_SRC_PAD);
+    const size_t src_bytes = sizeof(float) * (size_t)e->src_layer * V;
+    ALLOC(e->src_raw, src_bytes);
+    ALLOC(e->src, src_bytes);
+    HIPCHK_E(hipMemsetAsync(e->src_raw, 0, src_bytes, s));
+    e->ref_pitch = round_up(W, 64);
+    const size_t plane = sizeof(float) * (size_t)e->ref_pitch * H;
+    ALLOC(e->ref_raw, plane);
+    ALLOC(e->ref, plane);
+    ALLOC(e->sum_I, plane);
+    ALLOC(e->sum_II, plane);
+    e->stats_pitch = e->ref_pitch;
+    ALLOC(e->stats, sizeof(float2) * (size_t)e->stats_pitch * H);
+
+    /* upload images */
+    {
+        void* staging = nullptr;
+        const size_t maxbytes = (size_t)std::max(W * H, maxW * maxH) * esz;
+        if (image_format == PMX_IMG_U8) ALLOC(staging, maxbytes);
+        auto upload = [&](const void* host, int hpitch, int w, int h, float* dst, int dld) -> hipError_t {
+            if (image_format == PMX_IMG_F32)
+                return hipMemcpy2DAsync(dst, (size_t)dld * 4, host, hpitch, (size_t)w * 4, h,
+                                        hipMemcpyHostToDevice, s);
+            hipError_t er = hipMemcpy2DAsync(staging, w, host, hpitch, w, h, hipMemcpyHostToDevice, s);
+            if (er != hipSuccess) return er;
+            er = pmx_launch_convert_u8(s, (const uint8_t*)staging, w, dst, dld, w, h);
+            if (er != hipSuccess) return er;
+            return hipStreamSynchronize(s);
+        };
+        hipError_t er = upload(ref_img, ref_pitch, W, H, e->ref_raw, e->ref_pitch);
+        for (int v = 0; v < V && er == hipSuccess; v++)
+            er = upload(src_imgs[v], src_pitch[v], src_widths[v], src_heights[v],
+                        e->src_raw + (size_t)v * e->src_layer + (size_t)PMX_SRC_PAD * e->src_pitch +
+                            PMX_SRC_PAD,
+                        e->src_pitch);
+        if (er == hipSuccess) er = hipStreamSynchronize(s);
+        dev_free(staging);
+        HIPCHK_E(er);
+    }
+
+    /* homographies: FP64 on the device */
+    {
+        ALLOC(e->Hm, sizeof(double) * 18 * V);
+        std::vector<double> cams((size_t)(V + 1) * 21);
+        auto pack = [&](const pmx_camera& c, double* o) {
+            std::memcpy(o, c.K, 9 * sizeof(double));
+            std::memcpy(o + 9, c.R, 9 * sizeof(double));
+            std::memcpy(o + 18, c.C, 3 * sizeof(double));
+        };
+        pack(*ref_cam, cams.data());
+        for (int v = 0; v < V; v++) pack(src_cams[v], cams.data() + (size_t)(v + 1) * 21);
+        double* dc = nullptr;
+        ALLOC(dc, cams.size() * sizeof(double));
+        hipError_t er = hipMemcpyAsync(dc, cams.data(), cams.size() * sizeof(double),
+                                       hipMemcpyHostToDevice, s);
+        if (er == hipSuccess)
+            er = pmx_launch_compose(s, dc, V, p.orientation_x, p.orientation_z, e->Hm);
+        e->Hhost.resize((size_t)V * 18);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(e->Hhost.data(), e->Hm, sizeof(double) * 18 * V,
+                                hipMemcpyDeviceToHost, s);
+        if (er == hipSuccess) er = hipStreamSynchronize(s);
+        dev_free(dc);
+        HIPCHK_E(er);
+        /* K^-1 entries (0,0) (0,2) (1,1) (1,2) of the reference
+         * (src/patchMatch.cu:154-157), FP64 */
+        const double* K = ref_cam->K;
+        const double a = K[0], b = K[1], c = K[2], d = K[3], ee = K[4], f = K[5], g = K[6],
+                     h = K[7], i = K[8];
+        const double A = ee * i - f * h, B = -(d * i - f * g), C = d * h - ee * g;
+        const double det = a * A + b * B + c * C;
+        if (!(std::fabs(det) > 0.0)) {
+            engine_free(e);
+            return fail(ctx, PMX_ERR_ARG, "reference K is singular");
+        }
+        const double r = 1.0 / det;
+        e->invK[0] = A * r;
+        e->invK[1] = (b * f - c * ee) * r;
+        e->invK[2] = (a * i - c * g) * r;
+        e->invK[3] = -(a * f - c * d) * r;
+    }
+
+    /* state */
+    e->depth_rm_ld = round_up(W, 64);
+    e->depth_cm_ld = round_up(H, 64);
+    e->cost_rm_ld = round_up(W * V, 64);
+    e->cost_cm_ld = round_up(H * V, 64);
+    ALLOC(e->depth_rm, sizeof(float) * (size_t)e->depth_rm_ld * H);
+    ALLOC(e->depth_cm, sizeof(float) * (size_t)e->depth_cm_ld * W);
+    ALLOC(e->cost_rm, sizeof(float) * (size_t)e->cost_rm_ld * H);
+    ALLOC(e->cost_cm, sizeof(float) * (size_t)e->cost_cm_ld * W);
+    /* messages; also the scratch plane of the pre-blur */
+    ALLOC(e->spm, std::max(sizeof(double) * (size_t)W * H * V, sizeof(float) * (size_t)maxW * maxH));
+    HIPCHK_E(hipMemsetAsync(e->cost_rm, 0, sizeof(float) * (size_t)e->cost_rm_ld * H, s));
+    e->rng_pitch = (int)(sizeof(pmx_rng_state) * (size_t)W);
+    ALLOC(e->rng, (size_t)e->rng_pitch * H);
+    e->ids_layers = p.num_samples;
+    e->ids_pitch = round_up(W, 64);
+    ALLOC(e->used_ids, (size_t)e->ids_pitch * H * e->ids_layers);
+    HIPCHK_E(hipMemsetAsync(e->used_ids, 0, (size_t)e->ids_pitch * H * e->ids_layers, s));
+    e->out_depth_pitch = (int)sizeof(float) * e->depth_rm_ld;
+    e->out_cost_pitch = (int)sizeof(float) * e->ref_pitch;
+    ALLOC(e->out_cost, (size_t)e->out_cost_pitch * H * V);
+    ALLOC(e->counters, sizeof(unsigned long long) * 4);
+    HIPCHK_E(hipMemsetAsync(e->counters, 0, sizeof(unsigned long long) * 4, s));
+    HIPCHK_E(hipStreamSynchronize(s));
+#undef ALLOC
+#undef HIPCHK_E
+    *out = e;
+    return PMX_OK;
+}
+
+int pmx_engine_destroy(pmx_engine* e) {
+    if (!e) return PMX_OK;
+    if (e->ctx && e->ctx->stream) (void)hipStreamSynchronize(e->ctx->stream);
+    engine_free(e);
+    return PMX_OK;
+}
+
+static int prepare_impl(pmx_engine* e, Timer* tm) {
+    pmx_ctx* ctx = e->ctx;
+    hipStream_t s = ctx->stream;
+    const int W = e->W, H = e->H, V = e->V;
+    const int id = tm ? tm->begin(K_PREPARE) : -1;
+    /* pre-blur: the reference and every source over its own extent; the
+     * padded block stays zero outside each image */
+    const size_t src_bytes = sizeof(float) * (size_t)e->src_layer * V;
+    HIPCHK(ctx, hipMemcpyAsync(e->src, e->src_raw, src_bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(e->ref, e->ref_raw, sizeof(float) * (size_t)e->ref_pitch * H,
+                               hipMemcpyDeviceToDevice, s));
+    if (e->p.blur_mode != PMX_BLUR_NONE) {
+        float taps[45];
+        const int n = blur_taps(e->p.gaussian_sigma, e->p.blur_mode, taps);
+        float* tmp = (float*)e->spm; /* >= max(W*H, maxW*maxH) floats */
+        HIPCHK(ctx, pmx_launch_blur(s, e->ref, e->ref_pitch, W, H, 1, 0, taps, n, tmp));
+        for (int v = 0; v < V; v++) {
+            const int w = e->src_w[v], h = e->src_h[v];
+            float* layer = e->src + (size_t)v * e->src_layer + (size_t)PMX_SRC_PAD * e->src_pitch + PMX_SRC_PAD;
+            HIPCHK(ctx, pmx_launch_blur(s, layer, e->src_pitch, w, h, 1, 0, taps, n, tmp));
+        }
+    }
+    /* reference box sums and NCC statistics (FP64) */
+    double* dtmp = (double*)e->cost_cm; /* >= 2*W*H doubles when V >= 4; else allocate */
+    void* own = nullptr;
+    if ((size_t)e->cost_cm_ld * W * sizeof(float) < sizeof(double) * 2 * (size_t)W * H) {
+        PMXCHK(dev_alloc(ctx, &own, sizeof(double) * 2 * (size_t)W * H));
+        dtmp = (double*)own;
+    }
+    hipError_t er = pmx_launch_box_sums(s, e->ref, e->ref_pitch, W, H, e->p.half_window, nullptr,
+                                        e->sum_I, e->sum_II, e->ref_pitch, e->stats, e->stats_pitch,
+                                        dtmp);
+    if (own) {
+        hipError_t e2 = hipStreamSynchronize(s);
+        dev_free(own);
+        if (er == hipSuccess) er = e2;
+    }
+    HIPCHK(ctx, er);
+    /* RNG + initial depth (src/patchMatch.cu:164-166) */
+    HIPCHK(ctx, pmx_launch_rng_init(s, e->rng, e->rng_pitch, W, H, e->p.block_x, e->p.block_y,
+                                    e->p.rng_seed));
+    HIPCHK(ctx, pmx_launch_fill_depth(s, e->rng, e->rng_pitch, e->depth_rm, e->depth_rm_ld, W, H,
+                                      e->p.depth_near, e->p.depth_far, e->p.init_mode));
+    if (tm) tm->end(id);
+    e->layout_cm = 0;
+    e->prepared = true;
+    return PMX_OK;
+}
+
+int pmx_engine_prepare(pmx_engine* e) {
+    PMXCHK(check_engine(e));
+    Timer tm{e, {}};
+    int rc = prepare_impl(e, &tm);
+    tm.flush();
+    return rc;
+}
+
+int pmx_engine_run(pmx_engine* e) {
+    PMXCHK(check_engine(e));
+    PMXCHK(require_prepared(e));
+    Timer tm{e, {}};
+    const int iters = e->p.num_iterations;
+    int rc = PMX_OK;
+    for (int i = 0; i < iters && rc == PMX_OK; i++) {
+        const int S = (i == 0) ? 1 : e->p.num_samples; /* src/patchMatch.cu:328-331 */
+        if (i == 0) rc = do_cost_all(e, 1, &tm);
+        if (rc == PMX_OK) rc = do_sweep(e, 1, 1, S, false, &tm);   /* left to right */
+        if (rc == PMX_OK) rc = do_sweep(e, 1, 0, S, false, &tm);   /* top to bottom */
+        if (rc == PMX_OK) rc = do_sweep(e, 0, 1, S, false, &tm);   /* right to left */
+        if (rc == PMX_OK) rc = do_sweep(e, 0, 0, S, i == iters - 1, &tm); /* bottom to top */
+    }
+    if (rc == PMX_OK) rc = to_layout(e, 0, &tm);
+    tm.flush();
+    if (rc == PMX_OK) HIPCHK(e->ctx, hipGetLastError());
+    return rc;
+}
+
+int pmx_engine_cost_all_views(pmx_engine* e, int rotated) {
+    PMXCHK(check_engine(e));
+    PMXCHK(require_prepared(e));
+    Timer tm{e, {}};
+    int rc = do_cost_all(e, rotated, &tm);
+    tm.flush();
+    return rc;
+}
+
+int pmx_engine_sweep(pmx_engine* e, int down, int rotated, int S, int record_ids) {
+    PMXCHK(check_engine(e));
+    PMXCHK(require_prepared(e));
+    if (S < 1 || S > PMX_MAX_SAMPLES) return fail(e->ctx, PMX_ERR_ARG, "invalid num_samples %d", S);
+    Timer tm{e, {}};
+    int rc = do_sweep(e, down ? 1 : 0, rotated ? 1 : 0, S, record_ids != 0, &tm);
+    tm.flush();
+    return rc;
+}
+
+int pmx_engine_get_depth(pmx_engine* e, float* host, int pitch) {
+    PMXCHK(check_engine(e));
+    if (!host || pitch < e->W * 4) return fail(e->ctx, PMX_ERR_ARG, "invalid host buffer");
+    PMXCHK(to_layout(e, 0, nullptr));
+    HIPCHK(e->ctx, hipMemcpy2DAsync(host, pitch, e->depth_rm, (size_t)e->depth_rm_ld * 4,
+                                    (size_t)e->W * 4, e->H, hipMemcpyDeviceToHost, e->ctx->stream));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return PMX_OK;
+}
+
+int pmx_engine_set_depth(pmx_engine* e, const float* host, int pitch) {
+    PMXCHK(check_engine(e));
+    if (!host || pitch < e->W * 4) return fail(e->ctx, PMX_ERR_ARG, "invalid host buffer");
+    PMXCHK(to_layout(e, 0, nullptr));
+    HIPCHK(e->ctx, hipMemcpy2DAsync(e->depth_rm, (size_t)e->depth_rm_ld * 4, host, pitch,
+                                    (size_t)e->W * 4, e->H, hipMemcpyHostToDevice, e->ctx->stream));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return PMX_OK;
+}
+
+int pmx_engine_get_costs(pmx_engine* e, float* host) {
+    PMXCHK(check_engine(e));
+    if (!host) return fail(e->ctx, PMX_ERR_ARG, "invalid host buffer");
+    PMXCHK(to_layout(e, 0, nullptr));
+    const int ld = e->out_cost_pitch / 4;
+    HIPCHK(e->ctx, pmx_launch_gather_costs(e->ctx->stream, e->cost_rm, e->cost_rm_ld, e->out_cost,
+                                           ld, e->W, e->H, e->V));
+    HIPCHK(e->ctx, hipMemcpy2DAsync(host, (size_t)e->W * 4, e->out_cost, e->out_cost_pitch,
+                                    (size_t)e->W * 4, (size_t)e->H * e->V, hipMemcpyDeviceToHost,
+                                    e->ctx->stream));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return PMX_OK;
+}
+
+int pmx_engine_set_costs(pmx_engine* e, const float* host) {
+    PMXCHK(check_engine(e));
+    if (!host) return fail(e->ctx, PMX_ERR_ARG, "invalid host buffer");
+    PMXCHK(to_layout(e, 0, nullptr));
+    const int ld = e->out_cost_pitch / 4;
+    HIPCHK(e->ctx, hipMemcpy2DAsync(e->out_cost, e->out_cost_pitch, host, (size_t)e->W * 4,
+                                    (size_t)e->W * 4, (size_t)e->H * e->V, hipMemcpyHostToDevice,
+                                    e->ctx->stream));
+    HIPCHK(e->ctx, pmx_launch_scatter_costs(e->ctx->stream, e->out_cost, ld, e->cost_rm,
+                                            e->cost_rm_ld, e->W, e->H, e->V));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return PMX_OK;
+}
+
+int pmx_engine_get_rng(pmx_engine* e, pmx_rng_state* host) {
+    PMXCHK(check_engine(e));
+    if (!host) return fail(e->ctx, PMX_ERR_ARG, "invalid host buffer");
+    HIPCHK(e->ctx, hipMemcpyAsync(host, e->rng, (size_t)e->rng_pitch * e->H, hipMemcpyDeviceToHost,
+                                  e->ctx->stream));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return PMX_OK;
+}
+
+int pmx_engine_set_rng(pmx_engine* e, const pmx_rng_state* host) {
+    PMXCHK(check_engine(e));
+    if (!host) return fail(e->ctx, PMX_ERR_ARG, "invalid host buffer");
+    HIPCHK(e->ctx, hipMemcpyAsync(e->rng, host, (size_t)e->rng_pitch * e->H, hipMemcpyHostToDevice,
+                                  e->ctx->stream));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return PMX_OK;
+}
+
+int pmx_engine_get_used_ids(pmx_engine* e, uint8_t* host) {
+    PMXCHK(check_engine(e));
+    if (!host) return fail(e->ctx, PMX_ERR_ARG, "invalid host buffer");
+    HIPCHK(e->ctx, hipMemcpy2DAsync(host, e->W, e->used_ids, e->ids_pitch, e->W,
+                                    (size_t)e->H * e->ids_layers, hipMemcpyDeviceToHost,
+                                    e->ctx->stream));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    return PMX_OK;
+}
+
+int pmx_engine_get_images(pmx_engine* e, float* ref_host, float* src_host) {
+    PMXCHK(check_engine(e));
+    hipStream_t s = e->ctx->stream;
+    if (ref_host)
+        HIPCHK(e->ctx, hipMemcpy2DAsync(ref_host, (size_t)e->W * 4, e->ref, (size_t)e->ref_pitch * 4,
+                                        (size_t)e->W * 4, e->H, hipMemcpyDeviceToHost, s));
+    if (src_host)
+        for (int v = 0; v < e->V; v++)
+            HIPCHK(e->ctx,
+                   hipMemcpy2DAsync(src_host + (size_t)v * e->maxW * e->maxH, (size_t)e->maxW * 4,
+                                    e->src + (size_t)v * e->src_layer +
+                                        (size_t)PMX_SRC_PAD * e->src_pitch + PMX_SRC_PAD,
+                                    (size_t)e->src_pitch * 4, (size_t)e->maxW * 4, e->maxH,
+                                    hipMemcpyDeviceToHost, s));
+    HIPCHK(e->ctx, hipStreamSynchronize(s));
+    return PMX_OK;
+}
+
+int pmx_engine_get_ref_sums(pmx_engine* e, float* sum_I, float* sum_II) {
+    PMXCHK(check_engine(e));
+    hipStream_t s = e->ctx->stream;
+    if (sum_I)
+        HIPCHK(e->ctx, hipMemcpy2DAsync(sum_I, (size_t)e->W * 4, e->sum_I, (size_t)e->ref_pitch * 4,
+                                        (size_t)e->W * 4, e->H, hipMemcpyDeviceToHost, s));
+    if (sum_II)
+        HIPCHK(e->ctx, hipMemcpy2DAsync(sum_II, (size_t)e->W * 4, e->sum_II, (size_t)e->ref_pitch * 4,
+                                        (size_t)e->W * 4, e->H, hipMemcpyDeviceToHost, s));
+    HIPCHK(e->ctx, hipStreamSynchronize(s));
+    return PMX_OK;
+}
+
+int pmx_engine_get_homographies(pmx_engine* e, double* host) {
+    PMXCHK(check_engine(e));
+    if (!host) return fail(e->ctx, PMX_ERR_ARG, "invalid host buffer");
+    std::memcpy(host, e->Hhost.data(), e->Hhost.size() * sizeof(double));
+    return PMX_OK;
+}
+
+int pmx_engine_get_dims(pmx_engine* e, int* width, int* height, int* num_src, int* max_width,
+                        int* max_height) {
+    PMXCHK(check_engine(e));
+    if (width) *width = e->W;
+    if (height) *height = e->H;
+    if (num_src) *num_src = e->V;
+    if (max_width) *max_width = e->maxW;
+    if (max_height) *max_height = e->maxH;
+    return PMX_OK;
+}
+
+int pmx_engine_export(pmx_engine* e, float** depth, int* depth_pitch, float** costs,
+                      int* cost_pitch, uint8_t** used_ids, int* ids_pitch) {
+    PMXCHK(check_engine(e));
+    PMXCHK(to_layout(e, 0, nullptr));
+    HIPCHK(e->ctx, pmx_launch_gather_costs(e->ctx->stream, e->cost_rm, e->cost_rm_ld, e->out_cost,
+                                           e->out_cost_pitch / 4, e->W, e->H, e->V));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    if (depth) *depth = e->depth_rm;
+    if (depth_pitch) *depth_pitch = e->depth_rm_ld * 4;
+    if (costs) *costs = e->out_cost;
+    if (cost_pitch) *cost_pitch = e->out_cost_pitch;
+    if (used_ids) *used_ids = e->used_ids;
+    if (ids_pitch) *ids_pitch = e->ids_pitch;
+    return PMX_OK;
+}
+
+int pmx_engine_set_timing(pmx_engine* e, int enabled) {
+    PMXCHK(check_engine(e));
+    e->timing = enabled != 0;
+    return PMX_OK;
+}
+
+int pmx_engine_get_stats(pmx_engine* e, pmx_stats* out) {
+    PMXCHK(check_engine(e));
+    if (!out) return fail(e->ctx, PMX_ERR_ARG, "out is NULL");
+    unsigned long long c[4] = {0, 0, 0, 0};
+    HIPCHK(e->ctx, hipMemcpyAsync(c, e->counters, sizeof(c), hipMemcpyDeviceToHost, e->ctx->stream));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    *out = e->stats_acc;
+    out->ncc_evals = c[0];
+    out->ncc_samples = c[1];
+    out->cost_all_evals = c[2];
+    out->cost_all_samples = c[3];
+    return PMX_OK;
+}
+
+int pmx_engine_reset_stats(pmx_engine* e) {
+    PMXCHK(check_engine(e));
+    HIPCHK(e->ctx, hipMemsetAsync(e->counters, 0, sizeof(unsigned long long) * 4, e->ctx->stream));
+    HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+    e->stats_acc = pmx_stats{};
+    return PMX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,150 @@
+/*
+ * pmx_internal.h -- engine state and kernel launch interface shared by the
+ * HIP kernels (pmx_kernels.hip) and the C ABI (pmx_capi.hip).
+ *
+ * Data layout in HBM (one engine = one reference view against V sources):
+ *   sources   V layers, each (maxW + 4) x (maxH + 4) floats with a 2-pixel
+ *             zero border; an image smaller than maxW x maxH is zero-padded
+ *             like copyData (src/patchMatch.cu:72-95), so every bilinear tap
+ *             is a plain load and "border = 0" needs no test.
+ *   reference I (pre-blurred), sum I, sum I^2: W x H pitched float; per-pixel
+ *             (mean, clamped n*variance) as float2, computed in FP64.
+ *   state     depth (1 float) and V costs (V floats, view-innermost) per
+ *             pixel, kept column-contiguous for the sweep frame in use:
+ *             row-major for rotated (horizontal) sweeps, column-major for
+ *             plain (vertical) sweeps, moved between the two by LDS-tiled
+ *             transposes at the same points of the schedule as the
+ *             reference's transposeForward/Backward (src/patchMatch.cu:247-261).
+ *   messages  backward messages of the running sweep, V floats per pixel,
+ *             column-contiguous.
+ *   rng       one xorwow state per pixel (W x H, 24 bytes each); sweeps use
+ *             row 0 (plain) or column 0 (rotated), as src/patchMatch.cu:844-847.
+ */
+#ifndef PMX_INTERNAL_H
+#define PMX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pmx.h"
+
+#define PMX_SRC_PAD 2
+
+struct pmx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+/* Arguments of the sweep and cost kernels (device pointers). */
+struct PmxSweepArgs {
+    const float* src;      /* view 0, texel (0,0) of the padded source block */
+    long src_layer;        /* floats between layers */
+    int src_pitch;         /* floats between rows */
+    int srcW, srcH;        /* packed source size */
+    const float* ref;      /* pre-blurred reference, W x H */
+    int ref_pitch;         /* floats */
+    const float2* stats;   /* (mean, clamped n*variance) per pixel, W x H */
+    int stats_pitch;       /* float2 units */
+    const double* Hm;      /* V x 18: H1 then H2, row-major */
+    float* depth;          /* state: column c at depth + c*depth_ld */
+    int depth_ld;
+    float* cost;           /* state: column c at cost + c*cost_ld, [row][view] */
+    int cost_ld;
+    double* spm;           /* backward messages (FP64): column c at spm + c*spm_ld, [row][view] */
+    int spm_ld;
+    pmx_rng_state* rng;    /* W x H */
+    int rng_pitch;         /* bytes */
+    uint8_t* ids;          /* S planes of W x H (original frame) or null */
+    int ids_pitch;         /* bytes */
+    unsigned long long* counters; /* [0] NCC evaluations, [1] window samples */
+    int W, H, V, hw, S;
+    int Ws, Hs;            /* sweep frame */
+    int down, rotated;
+    float near_d, far_d;
+    double alpha2;          /* SPMAlpha^2 in FP64 */
+    double oX, oZ, iK0, iK1;
+};
+
+struct pmx_engine {
+    pmx_ctx* ctx = nullptr;
+    pmx_params p{};
+    int W = 0, H = 0, V = 0, maxW = 0, maxH = 0;
+    int image_format = PMX_IMG_F32;
+
+    /* images */
+    float* src_raw = nullptr;   /* padded source block as uploaded */
+    float* src = nullptr;       /* padded source block, pre-blurred */
+    long src_layer = 0;
+    int src_pitch = 0;          /* floats */
+    float* ref_raw = nullptr;   /* W x H, as uploaded */
+    float* ref = nullptr;       /* W x H, pre-blurred (public _refImageData) */
+    int ref_pitch = 0;          /* floats */
+    float* sum_I = nullptr;
+    float* sum_II = nullptr;
+    float2* stats = nullptr;
+    int stats_pitch = 0;        /* float2 units */
+
+    double* Hm = nullptr;       /* V x 18 */
+    std::vector<double> Hhost;
+    pmx_camera ref_cam{};
+    std::vector<pmx_camera> src_cams;
+    std::vector<int> src_w, src_h;
+    double invK[4] = {0, 0, 0, 0};
+
+    /* state */
+    float* depth_rm = nullptr;  int depth_rm_ld = 0;   /* row-major, ld in floats */
+    float* depth_cm = nullptr;  int depth_cm_ld = 0;   /* column-major */
+    float* cost_rm = nullptr;   int cost_rm_ld = 0;
+    float* cost_cm = nullptr;   int cost_cm_ld = 0;
+    double* spm = nullptr;      int spm_ld = 0;
+    int layout_cm = 0;          /* 1: state currently in the column-major buffers */
+
+    pmx_rng_state* rng = nullptr;
+    int rng_pitch = 0;          /* bytes */
+    uint8_t* used_ids = nullptr;
+    int ids_pitch = 0;          /* bytes */
+    int ids_layers = 0;
+
+    /* reference-layout exports */
+    float* out_depth = nullptr; int out_depth_pitch = 0;
+    float* out_cost = nullptr;  int out_cost_pitch = 0;
+
+    unsigned long long* counters = nullptr;
+    bool prepared = false;
+
+    /* timing */
+    bool timing = false;
+    pmx_stats stats_acc{};
+    std::vector<hipEvent_t> ev_pool;
+};
+
+/* ---- kernel launchers (pmx_kernels.hip); all return hipError_t ---- */
+hipError_t pmx_launch_transpose_f32(hipStream_t s, const float* in, float* out, int width,
+                                    int height, int ld_in, int ld_out);
+hipError_t pmx_launch_transpose_rec(hipStream_t s, const float* in, float* out, int width,
+                                    int height, int E, int ld_in, int ld_out);
+hipError_t pmx_launch_rng_init(hipStream_t s, pmx_rng_state* st, int pitch_bytes, int width,
+                               int height, int bx, int by, uint32_t seed);
+hipError_t pmx_launch_fill_depth(hipStream_t s, pmx_rng_state* st, int st_pitch_bytes,
+                                 float* out, int out_ld, int width, int height, float near_d,
+                                 float far_d, int mode);
+hipError_t pmx_launch_blur(hipStream_t s, float* data, int ld, int width, int height, int depth,
+                           long layer_stride, const float* taps, int ntaps, float* tmp);
+hipError_t pmx_launch_box_sums(hipStream_t s, const float* img, int img_ld, int width, int height,
+                               int hw, float* out_I, float* out_sI, float* out_sII, int out_ld,
+                               float2* stats, int stats_ld, double* tmp);
+hipError_t pmx_launch_compose(hipStream_t s, const double* cams /* (V+1)*21 */, int V, double oX,
+                              double oZ, double* H_out);
+hipError_t pmx_launch_convert_u8(hipStream_t s, const uint8_t* in, int in_pitch, float* out,
+                                 int out_ld, int width, int height);
+hipError_t pmx_launch_cost_all(hipStream_t s, const PmxSweepArgs& a);
+hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a);
+hipError_t pmx_launch_gather_costs(hipStream_t s, const float* cost_rm, int cost_ld, float* out,
+                                   int out_ld, int width, int height, int V);
+hipError_t pmx_launch_scatter_costs(hipStream_t s, const float* in, int in_ld, float* cost_rm,
+                                    int cost_ld, int width, int height, int V);
+
+#endif

@@ -1,0 +1,1017 @@
+/*
+ * pmx_kernels.hip -- gfx950 kernels of the PatchMatch engine.
+ *
+ * Hot path: pmx_sweep_kernel (the row-sequential propagation sweep with HMM
+ * view selection, src/patchMatch.cu:818-1209) and pmx_cost_all_kernel
+ * (src/patchMatch.cu:775-816).  Both evaluate the windowed NCC of
+ * src/patchMatch.cu:433-611 with one 16/32/64-lane group per (view, depth)
+ * hypothesis: the lanes of a group own one window column each and walk the
+ * window rows, so the bilinear taps of neighbouring lanes fall on
+ * neighbouring source texels.  A wavefront owns one sweep column and walks
+ * its rows; the per-pixel plane-induced homography is composed in FP64 and
+ * the window is projected relative to its centre, so FP32 only carries
+ * small offsets.  Window sums are accumulated in FP32 around the reference
+ * window mean (no cancellation in the variance), and the reference window
+ * itself (minus its mean) sits in LDS, prefetched one row ahead.
+ *
+ * Support kernels: LDS-tiled transposes, xorwow RNG init and depth fill,
+ * separable pre-blur, FP64 box sums, FP64 homography composition.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pmx_internal.h"
+
+namespace {
+
+constexpr int kLanes = 64;
+constexpr float kSentinel = 2.0f;       /* src/patchMatch.cu:594-595 */
+constexpr float kFix = 0.999f;          /* src/patchMatch.cu:9 */
+constexpr float kChange = 1.0f - 0.999f; /* src/patchMatch.cu:10 */
+constexpr float kUniformX = 0.83f;      /* src/patchMatch.cu:673 */
+
+/* ------------------------------------------------------------------ RNG */
+/* Marsaglia xorwow (J. Stat. Softw. 8(14), 2003); seeding by SplitMix64
+ * (Steele, Lea, Flood, OOPSLA 2014) of (seed << 32 | id). */
+struct Xw {
+    uint32_t x, y, z, w, v, d;
+};
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ Xw xw_seed(uint32_t id, uint32_t seed) {
+    uint64_t s = ((uint64_t)seed << 32) | (uint64_t)id;
+    const uint64_t a = splitmix64(s), b = splitmix64(s), c = splitmix64(s);
+    Xw st{(uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32), (uint32_t)c,
+          (uint32_t)(c >> 32)};
+    if ((st.x | st.y | st.z | st.w | st.v) == 0u) st.x = 1u;
+    return st;
+}
+
+__device__ __forceinline__ uint32_t xw_next(Xw& s) {
+    const uint32_t t = s.x ^ (s.x >> 2);
+    s.x = s.y;
+    s.y = s.z;
+    s.z = s.w;
+    s.w = s.v;
+    s.v = (s.v ^ (s.v << 4)) ^ (t ^ (t << 1));
+    s.d += 362437u;
+    return s.d + s.v;
+}
+
+/* ((r >> 8) + 1) * 2^-24: exact in FP32, range (0, 1]. */
+__device__ __forceinline__ float xw_uniform(Xw& s) {
+    return (float)((xw_next(s) >> 8) + 1u) * 5.9604644775390625e-08f;
+}
+
+__device__ __forceinline__ Xw xw_load(const pmx_rng_state* p) {
+    return Xw{p->x, p->y, p->z, p->w, p->v, p->d};
+}
+
+__device__ __forceinline__ void xw_store(pmx_rng_state* p, const Xw& s) {
+    p->x = s.x;
+    p->y = s.y;
+    p->z = s.z;
+    p->w = s.w;
+    p->v = s.v;
+    p->d = s.d;
+}
+
+__device__ __forceinline__ Xw xw_uniformize(Xw s) {
+    s.x = __builtin_amdgcn_readfirstlane(s.x);
+    s.y = __builtin_amdgcn_readfirstlane(s.y);
+    s.z = __builtin_amdgcn_readfirstlane(s.z);
+    s.w = __builtin_amdgcn_readfirstlane(s.w);
+    s.v = __builtin_amdgcn_readfirstlane(s.v);
+    s.d = __builtin_amdgcn_readfirstlane(s.d);
+    return s;
+}
+
+/* block-linear thread id of src/array2d_psng.cu:8-9 */
+__device__ __forceinline__ uint32_t seed_id(int x, int y, int width, int bx, int by) {
+    const int gx = (width - 1) / bx + 1;
+    return (uint32_t)(((y / by) * gx + (x / bx)) * bx * by + (y % by) * bx + (x % bx));
+}
+
+__global__ void rng_init_kernel(pmx_rng_state* st, int pitch, int width, int height, int bx,
+                                int by, uint32_t seed) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    pmx_rng_state* p = (pmx_rng_state*)((char*)st + (size_t)y * pitch) + x;
+    xw_store(p, xw_seed(seed_id(x, y, width, bx, by), seed));
+}
+
+/* generate_kernel_float (src/array2d_wrapper.cu:3-20). */
+__global__ void fill_depth_kernel(pmx_rng_state* st, int pitch, float* out, int ld, int width,
+                                  int height, float near_d, float far_d, int mode) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    pmx_rng_state* p = (pmx_rng_state*)((char*)st + (size_t)y * pitch) + x;
+    Xw s = xw_load(p);
+    const float u = xw_uniform(s);
+    float d;
+    if (mode == PMX_INIT_INVERSE) {
+        const float lo = 1.0f / far_d, hi = 1.0f / near_d;
+        d = 1.0f / ((1.0f - u) * (hi - lo) + lo);
+    } else {
+        d = (1.0f - u) * (far_d - near_d) + near_d;
+    }
+    out[(size_t)y * ld + x] = d;
+    xw_store(p, s);
+}
+
+/* ----------------------------------------------------------- transposes */
+/* Plane transpose, 32x32 tile, 32x8 threads; out[x][y] = in[y][x]. */
+__global__ __launch_bounds__(256) void transpose_f32_kernel(const float* __restrict__ in,
+                                                            float* __restrict__ out, int width,
+                                                            int height, int ld_in, int ld_out) {
+    __shared__ float tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int x = bx + tx, y = by + ty + k;
+        if (x < width && y < height) tile[ty + k][tx] = in[(size_t)y * ld_in + x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int y = by + tx, x = bx + ty + k; /* out row = x, out column = y */
+        if (x < width && y < height) out[(size_t)x * ld_out + y] = tile[tx][ty + k];
+    }
+}
+
+/* Transpose of E-float records: in[y][x][e] -> out[x][y][e]; T x T records
+ * per workgroup (T = 16, or 8 for E > 32 to bound LDS). */
+__global__ __launch_bounds__(256) void transpose_rec_kernel(const float* __restrict__ in,
+                                                            float* __restrict__ out, int width,
+                                                            int height, int E, int T, int ld_in,
+                                                            int ld_out) {
+    extern __shared__ float tile[]; /* T rows x (T*E + 1) */
+    const int row_len = T * E + 1;
+    const int bx = blockIdx.x * T, by = blockIdx.y * T;
+    const int t = threadIdx.x;
+    const int nx = min(T, width - bx), ny = min(T, height - by);
+    const int span_in = nx * E;
+    for (int i = t; i < T * T * E; i += 256) {
+        const int r = i / (T * E), f = i % (T * E);
+        if (r < ny && f < span_in) tile[r * row_len + f] = in[(size_t)(by + r) * ld_in + (size_t)bx * E + f];
+    }
+    __syncthreads();
+    const int span_out = ny * E;
+    for (int i = t; i < T * T * E; i += 256) {
+        const int xr = i / (T * E), f = i % (T * E);
+        if (xr < nx && f < span_out) {
+            const int yy = f / E, e = f % E;
+            out[(size_t)(bx + xr) * ld_out + (size_t)by * E + f] = tile[yy * row_len + xr * E + e];
+        }
+    }
+}
+
+/* Reference layout [v][y][x] <-> engine layout [y][x][v]. */
+__global__ void gather_costs_kernel(const float* __restrict__ cost, int cost_ld,
+                                    float* __restrict__ out, int out_ld, int width, int height,
+                                    int V) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    for (int v = 0; v < V; v++)
+        out[((size_t)v * height + y) * out_ld + x] = cost[(size_t)y * cost_ld + (size_t)x * V + v];
+}
+
+__global__ void scatter_costs_kernel(const float* __restrict__ in, int in_ld,
+                                     float* __restrict__ cost, int cost_ld, int width, int height,
+                                     int V) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    for (int v = 0; v < V; v++)
+        cost[(size_t)y * cost_ld + (size_t)x * V + v] = in[((size_t)v * height + y) * in_ld + x];
+}
+
+/* --------------------------------------------------------- image prep */
+struct Taps {
+    float t[45];
+};
+
+/* Separable filter with clamp-to-edge (src/GaussianBlurCUDA.cu:39-70):
+ * row pass into tmp, column pass back. */
+__global__ void blur_rows_kernel(const float* __restrict__ in, float* __restrict__ tmp, int ld,
+                                 int width, int height, Taps taps, int fr) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    float s = 0.0f;
+    for (int k = -fr; k <= fr; k++) {
+        const int xx = min(max(x + k, 0), width - 1);
+        s = fmaf(in[(size_t)y * ld + xx], taps.t[fr - k], s);
+    }
+    tmp[(size_t)y * width + x] = s;
+}
+
+__global__ void blur_cols_kernel(const float* __restrict__ tmp, float* __restrict__ out, int ld,
+                                 int width, int height, Taps taps, int fr) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    float s = 0.0f;
+    for (int k = -fr; k <= fr; k++) {
+        const int yy = min(max(y + k, 0), height - 1);
+        s = fmaf(tmp[(size_t)yy * width + x], taps.t[fr - k], s);
+    }
+    out[(size_t)y * ld + x] = s;
+}
+
+/* Box sums in FP64 (src/array2d_refImg.cu:11-67): row pass, then column
+ * pass which also writes the NCC statistics (mean, n*variance clamped at
+ * 1e-5 as src/patchMatch.cu:597-600). */
+__global__ void box_rows_kernel(const float* __restrict__ img, int ld, int width, int height,
+                                int hw, double2* __restrict__ tmp) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    double a = 0.0, b = 0.0;
+    const int x0 = max(x - hw, 0), x1 = min(x + hw, width - 1);
+    for (int xx = x0; xx <= x1; xx++) {
+        const double v = (double)img[(size_t)y * ld + xx];
+        a += v;
+        b = fma(v, v, b);
+    }
+    tmp[(size_t)y * width + x] = make_double2(a, b);
+}
+
+__global__ void box_cols_kernel(const double2* __restrict__ tmp, const float* __restrict__ img,
+                                int img_ld, int width, int height, int hw,
+                                float* __restrict__ out_I, float* __restrict__ out_sI,
+                                float* __restrict__ out_sII, int out_ld,
+                                float2* __restrict__ stats, int stats_ld) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    double a = 0.0, b = 0.0;
+    const int y0 = max(y - hw, 0), y1 = min(y + hw, height - 1);
+    for (int yy = y0; yy <= y1; yy++) {
+        const double2 t = tmp[(size_t)yy * width + x];
+        a += t.x;
+        b += t.y;
+    }
+    const size_t o = (size_t)y * out_ld + x;
+    if (out_I) out_I[o] = img[(size_t)y * img_ld + x];
+    if (out_sI) out_sI[o] = (float)a;
+    if (out_sII) out_sII[o] = (float)b;
+    if (stats) {
+        const int nx = min(x + hw, width - 1) - max(x - hw, 0) + 1;
+        const int ny = y1 - y0 + 1;
+        const double n = (double)(nx * ny);
+        double c1 = b - a * a / n;
+        c1 = c1 < 0.00001 ? 0.0 : c1;
+        stats[(size_t)y * stats_ld + x] = make_float2((float)(a / n), (float)c1);
+    }
+}
+
+__global__ void convert_u8_kernel(const uint8_t* __restrict__ in, int in_pitch,
+                                  float* __restrict__ out, int out_ld, int width, int height) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= width || y >= height) return;
+    out[(size_t)y * out_ld + x] = (float)in[(size_t)y * in_pitch + x] * (1.0f / 255.0f);
+}
+
+/* Image::init_relative (src/image.cpp:4-13) in FP64, one thread per view:
+ * H1 = Ks Rs Rr^T Kr^-1, H2 = Ks (Rs (Cs - Cr)) n^T Kr^-1, n = (oX, 0, oZ). */
+__device__ void m3mul(const double* a, const double* b, double* o) {
+    double t[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            t[i * 3 + j] = fma(a[i * 3 + 0], b[0 * 3 + j],
+                               fma(a[i * 3 + 1], b[1 * 3 + j], a[i * 3 + 2] * b[2 * 3 + j]));
+    for (int k = 0; k < 9; k++) o[k] = t[k];
+}
+
+__global__ void compose_kernel(const double* __restrict__ cams, int V, double oX, double oZ,
+                               double* __restrict__ Hout) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const double* Kr = cams;
+    const double* Rr = cams + 9;
+    const double* Cr = cams + 18;
+    const double* Ks = cams + (size_t)(v + 1) * 21;
+    const double* Rs = Ks + 9;
+    const double* Cs = Ks + 18;
+    /* Kr^-1 by cofactors */
+    const double a = Kr[0], b = Kr[1], c = Kr[2], d = Kr[3], e = Kr[4], f = Kr[5], g = Kr[6],
+                 h = Kr[7], i = Kr[8];
+    const double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    const double r = 1.0 / (a * A + b * B + c * C);
+    const double Ki[9] = {A * r, -(b * i - c * h) * r, (b * f - c * e) * r,
+                          B * r, (a * i - c * g) * r,  -(a * f - c * d) * r,
+                          C * r, -(a * h - b * g) * r, (a * e - b * d) * r};
+    double RrT[9];
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) RrT[p * 3 + q] = Rr[q * 3 + p];
+    double Rrel[9], t9[9];
+    m3mul(Rs, RrT, Rrel);
+    double tr[3];
+    for (int p = 0; p < 3; p++)
+        tr[p] = Rs[p * 3] * (Cs[0] - Cr[0]) + Rs[p * 3 + 1] * (Cs[1] - Cr[1]) +
+                Rs[p * 3 + 2] * (Cs[2] - Cr[2]);
+    double* H1 = Hout + (size_t)v * 18;
+    double* H2 = H1 + 9;
+    m3mul(Ks, Rrel, t9);
+    m3mul(t9, Ki, H1);
+    const double n[3] = {oX, 0.0, oZ};
+    double tn[9];
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) tn[p * 3 + q] = tr[p] * n[q];
+    m3mul(Ks, tn, t9);
+    m3mul(t9, Ki, H2);
+}
+
+/* ------------------------------------------------------------- NCC core */
+
+__device__ __forceinline__ int xcd_remap(int bid, int nwg) {
+    /* Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+     * contiguous run of columns so neighbouring columns share an L2. */
+    const int q = nwg >> 3, r = nwg & 7;
+    const int xcd = bid & 7, k = bid >> 3;
+    return (xcd < r) ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
+/* Per-pixel constants shared by every hypothesis of the pixel. */
+struct PixelCtx {
+    int x, y;            /* original frame */
+    int dy_lo, dy_hi;    /* window rows (0..Wn-1) whose reference row is inside */
+    bool lane_in;        /* this lane's window column is inside */
+    float mu, c1, D;
+    int npx;             /* in-bounds window pixels n */
+    double cx, cy;       /* window centre x + 0.5, y + 0.5 */
+    double scale;        /* n . K^-1 (x + 0.5, y + 0.5, 1) */
+};
+
+/* Projected window of one hypothesis, relative to the projected centre. */
+struct Proj {
+    float xcf, ycf;      /* centre in source texel space */
+    int ixc, iyc;        /* floor of the centre */
+    float fxc, fyc;      /* fractional part of the centre */
+    float ax, ay, bx, by, cx, cy;
+};
+
+__device__ __forceinline__ Proj make_proj(const double* __restrict__ Hv, float depth,
+                                          const PixelCtx& pc) {
+    /* T = H1 - H2 / depth / scale (src/patchMatch.cu:465-473), in FP64. */
+    const double inv = 1.0 / ((double)depth * pc.scale);
+    double h[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) h[k] = fma(-Hv[9 + k], inv, Hv[k]);
+    const double X0 = fma(h[0], pc.cx, fma(h[1], pc.cy, h[2]));
+    const double Y0 = fma(h[3], pc.cx, fma(h[4], pc.cy, h[5]));
+    const double Z0 = fma(h[6], pc.cx, fma(h[7], pc.cy, h[8]));
+    const double iz = 1.0 / Z0;
+    double xc = X0 * iz, yc = Y0 * iz;
+    xc = fmin(fmax(xc, -1.0e7), 1.0e7);
+    yc = fmin(fmax(yc, -1.0e7), 1.0e7);
+    const double flx = floor(xc), fly = floor(yc);
+    Proj p;
+    p.xcf = (float)xc;
+    p.ycf = (float)yc;
+    p.ixc = (int)flx;
+    p.iyc = (int)fly;
+    p.fxc = (float)(xc - flx);
+    p.fyc = (float)(yc - fly);
+    p.ax = (float)(h[0] * iz);
+    p.ay = (float)(h[1] * iz);
+    p.bx = (float)(h[3] * iz);
+    p.by = (float)(h[4] * iz);
+    p.cx = (float)(h[6] * iz);
+    p.cy = (float)(h[7] * iz);
+    return p;
+}
+
+/*
+ * One hypothesis per G-lane group: returns 1 - NCC in every lane of the
+ * group (src/patchMatch.cu:433-611).  `win` holds R' = Rq - mean for the
+ * window (row-major, Wn x Wn), Rq being the reference value the reference
+ * pairs with each sample (tile-index quirk already applied).
+ */
+template <int G>
+__device__ __forceinline__ float ncc_group(const PmxSweepArgs& a, const float* __restrict__ win,
+                                           int Wn, int gl, bool active, int view, float depth,
+                                           const PixelCtx& pc) {
+    float A = 0.0f, B = 0.0f, C = 0.0f, M = 0.0f;
+    /* Source samples are accumulated around K, the sample at the projected
+     * window centre, so B - A^2/n does not cancel when the source window's
+     * mean is far from the reference's (wrong-depth hypotheses). */
+    float K = 0.0f;
+    const float* __restrict__ base = a.src + (size_t)view * a.src_layer;
+    const int pitch = a.src_pitch;
+    const int maxx = a.srcW, maxy = a.srcH;
+    Proj pj{};
+    if (active) {
+        pj = make_proj(a.Hm + (size_t)view * 18, depth, pc);
+        const int gx = min(max(pj.ixc, -PMX_SRC_PAD), maxx);
+        const int gy = min(max(pj.iyc, -PMX_SRC_PAD), maxy);
+        const float* __restrict__ p = base + (ptrdiff_t)gy * pitch + gx;
+        const float t00 = p[0], t10 = p[1], t01 = p[pitch], t11 = p[pitch + 1];
+        const float top = fmaf(pj.fxc, t10 - t00, t00);
+        const float bot = fmaf(pj.fxc, t11 - t01, t01);
+        K = fmaf(pj.fyc, bot - top, top);
+    }
+    if (active && pc.lane_in) {
+        const float dxf = (float)(gl - a.hw);
+        const float axd = pj.ax * dxf, bxd = pj.bx * dxf, cxd = pj.cx * dxf;
+#pragma unroll 3
+        for (int dyi = pc.dy_lo; dyi <= pc.dy_hi; dyi++) {
+            const float dyf = (float)(dyi - a.hw);
+            const float dX = fmaf(pj.ay, dyf, axd);
+            const float dY = fmaf(pj.by, dyf, bxd);
+            const float dZ = fmaf(pj.cy, dyf, cxd);
+            const float rz = __builtin_amdgcn_rcpf(1.0f + dZ);
+            float sx = pj.fxc + fmaf(-pj.xcf, dZ, dX) * rz;
+            float sy = pj.fyc + fmaf(-pj.ycf, dZ, dY) * rz;
+            sx = fminf(fmaxf(sx, -1.0e7f), 1.0e7f);
+            sy = fminf(fmaxf(sy, -1.0e7f), 1.0e7f);
+            const float fx = floorf(sx), fy = floorf(sy);
+            const float al = sx - fx, be = sy - fy;
+            const int gx = min(max(pj.ixc + (int)fx, -PMX_SRC_PAD), maxx);
+            const int gy = min(max(pj.iyc + (int)fy, -PMX_SRC_PAD), maxy);
+            const float* __restrict__ p = base + (ptrdiff_t)gy * pitch + gx;
+            const float t00 = p[0], t10 = p[1], t01 = p[pitch], t11 = p[pitch + 1];
+            const float top = fmaf(al, t10 - t00, t00);
+            const float bot = fmaf(al, t11 - t01, t01);
+            const float val = fmaf(be, bot - top, top);
+            const float e = val - K;
+            const float rv = win[dyi * Wn + gl];
+            A += e;
+            B = fmaf(e, e, B);
+            C = fmaf(e, rv, C);
+            M = fmaxf(M, fabsf(val));
+        }
+    }
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) {
+        A += __shfl_xor(A, m, G);
+        B += __shfl_xor(B, m, G);
+        C += __shfl_xor(C, m, G);
+        M = fmaxf(M, __shfl_xor(M, m, G));
+    }
+    /* sum I'^2 == 0 <=> every in-bounds sample is 0 (src/patchMatch.cu:594-595) */
+    if (!(M > 0.0f)) return kSentinel;
+    /* Final normalisation in FP64 (src/patchMatch.cu:597-610):
+     *   n var' = B - A^2/n,  sum I' (Rq - mean) = C + K * D. */
+    const double n = (double)pc.npx;
+    double c2 = (double)B - (double)A * (double)A / n;
+    c2 = c2 < 0.00001 ? 0.0 : c2;
+    const double den = sqrt((double)pc.c1 * c2);
+    if (den == 0.0) return 1.0f;
+    return (float)(1.0 - ((double)C + (double)K * (double)pc.D) / den);
+}
+
+/* Loads the reference window of pixel (x, y) for one lane's share:
+ * positions idx = lane + 64*k.  Returns R' (or 0 outside) and adds the
+ * in-bounds values to *dsum. qx/qy: tile-index shift of the left (plain) or
+ * top (rotated) border (src/patchMatch.cu:536-551). */
+template <int PF>
+__device__ __forceinline__ void load_window(const PmxSweepArgs& a, int x, int y, int qx, int qy,
+                                            int Wn, float inv_wn, int lane, float* vals) {
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        const int idx = lane + kLanes * k;
+        float v = 0.0f;
+        if (idx < Wn * Wn) {
+            const int dyi = (int)(((float)idx + 0.5f) * inv_wn);
+            const int dxi = idx - dyi * Wn;
+            const int px = x + dxi - a.hw, py = y + dyi - a.hw;
+            if (px >= 0 && px < a.W && py >= 0 && py < a.H) {
+                const int sx = px - qx, sy = py - qy;
+                v = (sx >= 0 && sy >= 0) ? a.ref[(size_t)sy * a.ref_pitch + sx] : 0.0f;
+            } else {
+                v = __builtin_nanf("");
+            }
+        }
+        vals[k] = v;
+    }
+}
+
+/* Writes a loaded window as R' = Rq - mean into LDS; returns the wave sum of
+ * R' over the in-bounds positions. */
+template <int PF>
+__device__ __forceinline__ float store_window(float* __restrict__ win, const float* vals, float mu,
+                                              int Wn, int lane) {
+    float d = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        const int idx = lane + kLanes * k;
+        if (idx < Wn * Wn) {
+            const float v = vals[k];
+            const bool in = (v == v);
+            const float r = in ? v - mu : 0.0f;
+            win[idx] = r;
+            d += r;
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) d += __shfl_xor(d, m, kLanes);
+    return d;
+}
+
+__device__ __forceinline__ void pixel_ctx(const PmxSweepArgs& a, int x, int y, int gl,
+                                          PixelCtx& pc) {
+    const int hw = a.hw, Wn = 2 * hw + 1;
+    pc.x = x;
+    pc.y = y;
+    pc.dy_lo = max(0, hw - y);
+    pc.dy_hi = min(Wn - 1, a.H - 1 - y + hw);
+    const int px = x + gl - hw;
+    pc.lane_in = (gl < Wn) && px >= 0 && px < a.W;
+    const int nx = min(x + hw, a.W - 1) - max(x - hw, 0) + 1;
+    const int ny = pc.dy_hi - pc.dy_lo + 1;
+    pc.npx = nx * ny;
+    pc.cx = (double)x + 0.5;
+    pc.cy = (double)y + 0.5;
+    pc.scale = a.oX * (a.iK0 * pc.cx + a.iK1) + a.oZ;
+}
+
+/* ------------------------------------------------------ cost all views */
+/* computeAllCostGivenDepth (src/patchMatch.cu:775-816): one wavefront per
+ * pixel, G-lane groups over views.  Writes the row-major state. */
+template <int G, int PF>
+__global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
+    extern __shared__ float win[];
+    const int lane = threadIdx.x;
+    const int gl = lane % G, grp = lane / G;
+    constexpr int NG = kLanes / G;
+    const long pid = (long)blockIdx.x;
+    const int x = (int)(pid % a.W), y = (int)(pid / a.W);
+    const int Wn = 2 * a.hw + 1;
+    const float inv_wn = 1.0f / (float)Wn;
+    PixelCtx pc;
+    pixel_ctx(a, x, y, gl, pc);
+    const float2 st = a.stats[(size_t)y * a.stats_pitch + x];
+    pc.mu = st.x;
+    pc.c1 = st.y;
+    /* computeAllCostGivenDepth runs in the rotated frame: the tile shift is
+     * along y (src/patchMatch.cu:794). */
+    const int qx = a.rotated ? 0 : max(0, a.hw - x);
+    const int qy = a.rotated ? max(0, a.hw - y) : 0;
+    float vals[PF];
+    load_window<PF>(a, x, y, qx, qy, Wn, inv_wn, lane, vals);
+    pc.D = store_window<PF>(win, vals, pc.mu, Wn, lane);
+    /* Without the tile-index shift the in-bounds reference values sum to the
+     * box sum exactly, so D = sum (Rq - mean) is 0; use the exact value. */
+    if (qx == 0 && qy == 0) pc.D = 0.0f;
+    __syncthreads();
+    /* state is row-major: column c = y, row r = x */
+    const float depth = a.depth[(size_t)y * a.depth_ld + x];
+    float* cst = a.cost + (size_t)y * a.cost_ld + (size_t)x * a.V;
+    for (int v0 = 0; v0 < a.V; v0 += NG) {
+        const int v = v0 + grp;
+        const bool act = v < a.V;
+        const float c = ncc_group<G>(a, win, Wn, gl, act, act ? v : 0, depth, pc);
+        if (act && gl == 0) cst[v] = c;
+    }
+    if (lane == 0 && a.counters) {
+        const int nx = pc.npx;
+        atomicAdd(&a.counters[2], (unsigned long long)a.V);
+        atomicAdd(&a.counters[3], (unsigned long long)a.V * (unsigned long long)nx);
+    }
+}
+
+/* ----------------------------------------------------------- messages */
+/* The view-selection HMM runs in FP64: the reference evaluates the emission
+ * exp(-0.5 c^2 / alpha^2) in double (src/patchMatch.cu:683, :708), and the
+ * costs that the left/top-border tile-index quirk produces (|c| up to ~100)
+ * drive emissions far below the FP32 range, where FP32 messages would
+ * collapse to exactly 0.  Constants keep their FP32 literal values. */
+struct MsgConst {
+    double up;      /* exp(-0.5 x^2 / alpha^2), x = 0.83f */
+    double inv2a2;  /* 0.5 / alpha^2 */
+};
+
+__device__ __forceinline__ MsgConst msg_const(double alpha2) {
+    MsgConst mc;
+    const double x = (double)kUniformX;
+    mc.inv2a2 = 0.5 / alpha2;
+    mc.up = exp(-0.5 * x * x / alpha2);
+    return mc;
+}
+
+/* Sentinel draws: every view whose stored cost is exactly 2.0 consumes one
+ * uniform, in view order (src/patchMatch.cu:680-682, :706-707). */
+__device__ __forceinline__ double sentinel_emission(float cost, bool is_view, int lane, Xw& rng) {
+    const bool sent = is_view && (cost == kSentinel);
+    const uint64_t mask = __ballot(sent);
+    const int k = __popcll(mask);
+    double e = (double)cost;
+    if (k) {
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        for (int i = 0; i < k; i++) {
+            const float u = xw_uniform(rng);
+            if (sent && rank == i) e = (double)u * 2.0;
+        }
+    }
+    return e;
+}
+
+/* computeMessageBackward (src/patchMatch.cu:669-690) */
+__device__ __forceinline__ double msg_backward(double m, double e, const MsgConst& mc) {
+    const double fix = (double)kFix, chg = (double)kChange;
+    const double em = exp(-e * e * mc.inv2a2);
+    const double z0 = m * em * fix + (1.0 - m) * mc.up * chg;
+    const double z1 = m * em * chg + (1.0 - m) * mc.up * fix;
+    return z0 / (z0 + z1);
+}
+
+/* computeMessageForward (src/patchMatch.cu:692-715) */
+__device__ __forceinline__ double msg_forward(double m, double e, const MsgConst& mc) {
+    const double fix = (double)kFix, chg = (double)kChange;
+    const double em = exp(-e * e * mc.inv2a2);
+    const double z0 = (m * fix + (1.0 - m) * chg) * em;
+    const double z1 = (m * chg + (1.0 - m) * fix) * mc.up;
+    return z0 / (z0 + z1);
+}
+
+/* ---------------------------------------------------------------- sweep */
+/*
+ * topToDown / downToTop (src/patchMatch.cu:818-1209).  One wavefront per
+ * sweep column.  Lanes 0..V-1 carry the per-view HMM state; for each row the
+ * sampled views are deduplicated and the (view, depth) hypotheses are
+ * evaluated four (G=16) at a time.  The NCC of a view at the winning depth
+ * is reused when the view was sampled (the reference recomputes the same
+ * value).
+ */
+template <int G, int PF>
+__global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
+    extern __shared__ float smem[];
+    const int Wn = 2 * a.hw + 1;
+    const int WW = Wn * Wn;
+    float* winbuf = smem;                 /* 2 x WW */
+    float* tres = smem + 2 * WW;          /* task results, 2*64 */
+    int* tview = (int*)(tres + 128);      /* task views, 128 */
+    float* tdep = (float*)(tview + 128);  /* task depths, 128 */
+    int* vj = (int*)(tdep + 128);         /* sampled views, S */
+
+    constexpr int NG = kLanes / G;
+    const int lane = threadIdx.x;
+    const int gl = lane % G, grp = lane / G;
+    const int c = xcd_remap(blockIdx.x, gridDim.x);
+    if (c >= a.Ws) return;
+    const int V = a.V, Hs = a.Hs;
+    const bool is_view = lane < V;
+    const int vl = is_view ? lane : 0;
+    const float inv_wn = 1.0f / (float)Wn;
+
+    const MsgConst mc = msg_const(a.alpha2);
+
+    float* dcol = a.depth + (size_t)c * a.depth_ld;
+    float* ccol = a.cost + (size_t)c * a.cost_ld;
+    double* scol = a.spm + (size_t)c * a.spm_ld;
+
+    /* generator of this column: row 0 (plain) or column 0 (rotated) */
+    pmx_rng_state* rp = a.rotated ? (pmx_rng_state*)((char*)a.rng + (size_t)c * a.rng_pitch)
+                                  : a.rng + c;
+    Xw rng = xw_uniformize(xw_load(rp));
+
+    /* ---- backward messages (src/patchMatch.cu:858-869, :1050-1061) ---- */
+    double m = 0.5;
+    {
+        int rb = a.down ? Hs - 1 : 0;
+        const int sb = a.down ? -1 : 1;
+        float cn = ccol[(size_t)rb * V + vl];
+        for (int k = 0; k < Hs; k++, rb += sb) {
+            const float cst = cn;
+            if (k + 1 < Hs) cn = ccol[(size_t)(rb + sb) * V + vl];
+            if (is_view) scol[(size_t)rb * V + lane] = m;
+            const double e = sentinel_emission(cst, is_view, lane, rng);
+            m = msg_backward(m, e, mc);
+        }
+    }
+    /* the backward messages are re-read below by this wavefront */
+    __threadfence_block();
+    __syncthreads();
+    /* ---- forward message at the first row ---- */
+    const int step = a.down ? 1 : -1;
+    const int r0 = a.down ? 0 : Hs - 1;
+    {
+        const float cst = ccol[(size_t)r0 * V + vl];
+        const double e = sentinel_emission(cst, is_view, lane, rng);
+        m = msg_forward(0.5, e, mc);
+    }
+    float d_former = dcol[r0];
+
+    /* original-frame coordinates of sweep row r in this column */
+    auto orig_xy = [&](int r, int& x, int& y) {
+        if (a.rotated) {
+            x = r;
+            y = c;
+        } else {
+            x = c;
+            y = r;
+        }
+    };
+    const int qx_col = a.rotated ? 0 : max(0, a.hw - c);
+    const int qy_col = a.rotated ? max(0, a.hw - c) : 0;
+
+    /* prefetch row r0 + step */
+    int r = r0 + step;
+    float pf_cost = 0.0f, pf_depth = 0.0f;
+    double pf_beta = 0.0;
+    float2 pf_stat = make_float2(0.0f, 0.0f);
+    float pf_win[PF];
+    auto prefetch = [&](int rr) {
+        int x, y;
+        orig_xy(rr, x, y);
+        pf_cost = ccol[(size_t)rr * V + vl];
+        pf_beta = scol[(size_t)rr * V + vl];
+        pf_depth = dcol[rr];
+        pf_stat = a.stats[(size_t)y * a.stats_pitch + x];
+        load_window<PF>(a, x, y, qx_col, qy_col, Wn, inv_wn, lane, pf_win);
+    };
+    if (Hs > 1) prefetch(r);
+    int buf = 0;
+    float D = 0.0f;
+    if (Hs > 1) D = store_window<PF>(winbuf, pf_win, pf_stat.x, Wn, lane);
+
+    unsigned long long n_evals = 0, n_samples = 0;
+
+    for (int k = 1; k < Hs; k++, r += step) {
+        float* win = winbuf + buf * WW;
+        __syncthreads();
+        const float cst = pf_cost, d_cur = pf_depth;
+        const double beta = pf_beta;
+        const float2 st = pf_stat;
+        if (k + 1 < Hs) prefetch(r + step);
+
+        int x, y;
+        orig_xy(r, x, y);
+        PixelCtx pc;
+        pixel_ctx(a, x, y, gl, pc);
+        pc.mu = st.x;
+        pc.c1 = st.y;
+        /* D = sum (Rq - mean) is exactly 0 unless the tile-index shift
+         * applies to this column */
+        pc.D = (qx_col == 0 && qy_col == 0) ? 0.0f : D;
+        const int npx = pc.npx;
+
+        /* forward message with the stored costs, posterior, CDF
+         * (src/patchMatch.cu:901-923) */
+        const double m_prev = m;
+        double e = sentinel_emission(cst, is_view, lane, rng);
+        const double mf = msg_forward(m, e, mc);
+        double q = 0.0;
+        if (is_view) {
+            const double z0 = mf * beta;
+            const double z1 = (1.0 - mf) * (1.0 - beta);
+            q = z0 / (z0 + z1);
+        }
+        /* inclusive prefix sum over lanes 0..V-1 (in view order, as the
+         * sequential sum of src/patchMatch.cu:917-919) */
+        double cdf = q;
+#pragma unroll
+        for (int o = 1; o < kLanes; o <<= 1) {
+            const double t = __shfl_up(cdf, o, kLanes);
+            if (lane >= o) cdf += t;
+        }
+        const double total = __shfl(cdf, V - 1, kLanes);
+        cdf = (lane == V - 1) ? 1.0 : cdf / total;
+
+        /* draws: random depth, then S views (src/patchMatch.cu:937-961) */
+        const float d_rand = xw_uniform(rng) * (a.far_d - a.near_d) + a.near_d;
+        uint64_t sel = 0;
+        for (int j = 0; j < a.S; j++) {
+            const double u = (double)xw_uniform(rng);
+            const uint64_t hit = __ballot(is_view && u <= cdf);
+            const int id = hit ? (int)__builtin_ctzll(hit) : V - 1;
+            sel |= 1ull << id;
+            if (lane == 0) vj[j] = id;
+            if (a.ids && lane == 0)
+                a.ids[((size_t)j * a.H + y) * a.ids_pitch + x] = (uint8_t)id;
+        }
+        const int U = __popcll(sel);
+        const bool in_sel = is_view && ((sel >> lane) & 1ull);
+        const int srank = __popcll(sel & ((1ull << lane) - 1ull));
+        if (in_sel) {
+            tview[2 * srank] = lane;
+            tview[2 * srank + 1] = lane;
+            tdep[2 * srank] = d_former;
+            tdep[2 * srank + 1] = d_rand;
+        }
+        __syncthreads();
+
+        /* phase B: former and random depth on the sampled views */
+        const int nt = 2 * U;
+        for (int t0 = 0; t0 < nt; t0 += NG) {
+            const int t = t0 + grp;
+            const bool act = t < nt;
+            const int tv = act ? tview[t] : 0;
+            const float td = act ? tdep[t] : 1.0f;
+            const float cc = ncc_group<G>(a, win, Wn, gl, act, tv, td, pc);
+            if (act && gl == 0) tres[t] = cc;
+        }
+        __syncthreads();
+        float nf = 0.0f, nr = 0.0f;
+        if (in_sel) {
+            nf = tres[2 * srank];
+            nr = tres[2 * srank + 1];
+        }
+        float c0 = 0.0f, c1s = 0.0f, c2 = 0.0f;
+        for (int j = 0; j < a.S; j++) {
+            const int id = vj[j];
+            c0 += __shfl(nf, id, kLanes);
+            c1s += __shfl(cst, id, kLanes);
+            c2 += __shfl(nr, id, kLanes);
+        }
+        const float inv_s = 1.0f / (float)a.S;
+        c0 *= inv_s;
+        c1s *= inv_s;
+        c2 *= inv_s;
+        /* findMinCost (src/patchMatch.cu:613-618) */
+        int idx = c0 < c1s ? 0 : 1;
+        const float cidx = idx == 0 ? c0 : c1s;
+        idx = cidx < c2 ? idx : 2;
+        const float best = idx == 0 ? d_former : (idx == 1 ? d_cur : d_rand);
+        if (lane == 0) dcol[r] = best;
+        n_evals += (unsigned long long)nt;
+
+        float cnew = cst;
+        if (idx != 1) {
+            /* phase C: remaining views at the winning depth */
+            const uint64_t vmask = (V >= 64) ? ~0ull : ((1ull << V) - 1ull);
+            const uint64_t rest = vmask & ~sel;
+            const int nrest = V - U;
+            const bool in_rest = is_view && !in_sel;
+            const int rrank = __popcll(rest & ((1ull << lane) - 1ull));
+            if (in_rest) {
+                tview[rrank] = lane;
+                tdep[rrank] = best;
+            }
+            __syncthreads();
+            for (int t0 = 0; t0 < nrest; t0 += NG) {
+                const int t = t0 + grp;
+                const bool act = t < nrest;
+                const int tv = act ? tview[t] : 0;
+                const float cc = ncc_group<G>(a, win, Wn, gl, act, tv, best, pc);
+                if (act && gl == 0) tres[64 + t] = cc;
+            }
+            __syncthreads();
+            if (in_sel) cnew = (idx == 0) ? nf : nr;
+            if (in_rest) cnew = tres[64 + rrank];
+            if (is_view) ccol[(size_t)r * V + lane] = cnew;
+            n_evals += (unsigned long long)nrest;
+        }
+        n_samples += (unsigned long long)npx * (unsigned long long)(nt + (idx != 1 ? V - U : 0));
+
+        /* forward message with the updated costs (src/patchMatch.cu:996-999) */
+        e = sentinel_emission(cnew, is_view, lane, rng);
+        m = msg_forward(m_prev, e, mc);
+        d_former = best;
+
+        /* next window into the other buffer */
+        if (k + 1 < Hs) {
+            buf ^= 1;
+            D = store_window<PF>(winbuf + buf * WW, pf_win, pf_stat.x, Wn, lane);
+        }
+    }
+    if (lane == 0) {
+        xw_store(rp, rng);
+        if (a.counters) {
+            atomicAdd(&a.counters[0], n_evals);
+            atomicAdd(&a.counters[1], n_samples);
+        }
+    }
+}
+
+inline dim3 grid2(int w, int h, dim3 b) { return dim3((w + b.x - 1) / b.x, (h + b.y - 1) / b.y); }
+
+} // namespace
+
+/* ----------------------------------------------------------- launchers */
+
+hipError_t pmx_launch_transpose_f32(hipStream_t s, const float* in, float* out, int width,
+                                    int height, int ld_in, int ld_out) {
+    dim3 b(32, 8), g((width + 31) / 32, (height + 31) / 32);
+    hipLaunchKernelGGL(transpose_f32_kernel, g, b, 0, s, in, out, width, height, ld_in, ld_out);
+    return hipGetLastError();
+}
+
+hipError_t pmx_launch_transpose_rec(hipStream_t s, const float* in, float* out, int width,
+                                    int height, int E, int ld_in, int ld_out) {
+    if (E == 1) return pmx_launch_transpose_f32(s, in, out, width, height, ld_in, ld_out);
+    const int T = E > 32 ? 8 : 16;
+    dim3 b(256), g((width + T - 1) / T, (height + T - 1) / T);
+    const size_t sh = sizeof(float) * T * (T * E + 1);
+    hipLaunchKernelGGL(transpose_rec_kernel, g, b, sh, s, in, out, width, height, E, T, ld_in,
+                       ld_out);
+    return hipGetLastError();
+}
+
+hipError_t pmx_launch_rng_init(hipStream_t s, pmx_rng_state* st, int pitch, int width, int height,
+                               int bx, int by, uint32_t seed) {
+    dim3 b(32, 8);
+    hipLaunchKernelGGL(rng_init_kernel, grid2(width, height, b), b, 0, s, st, pitch, width, height,
+                       bx, by, seed);
+    return hipGetLastError();
+}
+
+hipError_t pmx_launch_fill_depth(hipStream_t s, pmx_rng_state* st, int st_pitch, float* out,
+                                 int out_ld, int width, int height, float near_d, float far_d,
+                                 int mode) {
+    dim3 b(32, 8);
+    hipLaunchKernelGGL(fill_depth_kernel, grid2(width, height, b), b, 0, s, st, st_pitch, out,
+                       out_ld, width, height, near_d, far_d, mode);
+    return hipGetLastError();
+}
+
+hipError_t pmx_launch_blur(hipStream_t s, float* data, int ld, int width, int height, int depth,
+                           long layer_stride, const float* taps, int ntaps, float* tmp) {
+    Taps t{};
+    for (int i = 0; i < ntaps && i < 45; i++) t.t[i] = taps[i];
+    const int fr = ntaps >> 1;
+    dim3 b(32, 8), g = grid2(width, height, b);
+    for (int k = 0; k < depth; k++) {
+        float* layer = data + (size_t)k * layer_stride;
+        hipLaunchKernelGGL(blur_rows_kernel, g, b, 0, s, layer, tmp, ld, width, height, t, fr);
+        hipLaunchKernelGGL(blur_cols_kernel, g, b, 0, s, tmp, layer, ld, width, height, t, fr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t pmx_launch_box_sums(hipStream_t s, const float* img, int img_ld, int width, int height,
+                               int hw, float* out_I, float* out_sI, float* out_sII, int out_ld,
+                               float2* stats, int stats_ld, double* tmp) {
+    dim3 b(32, 8), g = grid2(width, height, b);
+    hipLaunchKernelGGL(box_rows_kernel, g, b, 0, s, img, img_ld, width, height, hw, (double2*)tmp);
+    hipLaunchKernelGGL(box_cols_kernel, g, b, 0, s, (const double2*)tmp, img, img_ld, width, height,
+                       hw, out_I, out_sI, out_sII, out_ld, stats, stats_ld);
+    return hipGetLastError();
+}
+
+hipError_t pmx_launch_compose(hipStream_t s, const double* cams, int V, double oX, double oZ,
+                              double* H_out) {
+    hipLaunchKernelGGL(compose_kernel, dim3((V + 63) / 64), dim3(64), 0, s, cams, V, oX, oZ, H_out);
+    return hipGetLastError();
+}
+
+hipError_t pmx_launch_convert_u8(hipStream_t s, const uint8_t* in, int in_pitch, float* out,
+                                 int out_ld, int width, int height) {
+    dim3 b(32, 8);
+    hipLaunchKernelGGL(convert_u8_kernel, grid2(width, height, b), b, 0, s, in, in_pitch, out,
+                       out_ld, width, height);
+    return hipGetLastError();
+}
+
+hipError_t pmx_launch_gather_costs(hipStream_t s, const float* cost, int cost_ld, float* out,
+                                   int out_ld, int width, int height, int V) {
+    dim3 b(32, 8);
+    hipLaunchKernelGGL(gather_costs_kernel, grid2(width, height, b), b, 0, s, cost, cost_ld, out,
+                       out_ld, width, height, V);
+    return hipGetLastError();
+}
+
+hipError_t pmx_launch_scatter_costs(hipStream_t s, const float* in, int in_ld, float* cost,
+                                    int cost_ld, int width, int height, int V) {
+    dim3 b(32, 8);
+    hipLaunchKernelGGL(scatter_costs_kernel, grid2(width, height, b), b, 0, s, in, in_ld, cost,
+                       cost_ld, width, height, V);
+    return hipGetLastError();
+}
+
+namespace {
+template <int G, int PF>
+hipError_t launch_cost_all_t(hipStream_t s, const PmxSweepArgs& a) {
+    const int Wn = 2 * a.hw + 1;
+    const size_t sh = sizeof(float) * Wn * Wn;
+    hipLaunchKernelGGL((pmx_cost_all_kernel<G, PF>), dim3((unsigned)((long)a.W * a.H)), dim3(64),
+                       sh, s, a);
+    return hipGetLastError();
+}
+
+template <int G, int PF>
+hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
+    const int Wn = 2 * a.hw + 1;
+    const size_t sh = sizeof(float) * (2 * Wn * Wn + 128 + 128 + 128 + 64);
+    hipLaunchKernelGGL((pmx_sweep_kernel<G, PF>), dim3(a.Ws), dim3(64), sh, s, a);
+    return hipGetLastError();
+}
+} // namespace
+
+hipError_t pmx_launch_cost_all(hipStream_t s, const PmxSweepArgs& a) {
+    const int Wn = 2 * a.hw + 1;
+    if (Wn <= 16) return launch_cost_all_t<16, 4>(s, a);
+    if (Wn <= 32) return launch_cost_all_t<32, 16>(s, a);
+    return launch_cost_all_t<64, 44>(s, a);
+}
+
+hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a) {
+    const int Wn = 2 * a.hw + 1;
+    if (Wn <= 16) return launch_sweep_t<16, 4>(s, a);
+    if (Wn <= 32) return launch_sweep_t<32, 16>(s, a);
+    return launch_sweep_t<64, 44>(s, a);
+}

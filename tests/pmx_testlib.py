@@ -1,0 +1,83 @@
+"""Shared helpers of the parity tests: build the oracle problem that sees the
+same inputs as a GPU engine, and compare sweeps margin-aware."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import pmx_oracle as O
+from patchmatch_cuda_amd import capi, configs, scene
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+def make_engine(ctx, spec, rp, prepare=True):
+    sc = scene.render(spec)
+    eng = capi.Engine(ctx, rp, sc.ref, list(sc.srcs), sc.cams)
+    if prepare:
+        eng.prepare()
+    return sc, eng
+
+
+def oracle_for(eng, rp, sc):
+    """Oracle problem fed with the engine's own pre-blurred images and FP64
+    homographies (teacher forcing of everything upstream of the NCC)."""
+    ref, srcs = eng.images()
+    return O.Problem(ref, srcs, eng.homographies(), rp.half_window, f32(rp.depth_near),
+                     f32(rp.depth_far), f32(rp.spm_alpha), O.inv_k4(sc.cams[0, :9]),
+                     f32(rp.orientation_x), f32(rp.orientation_z))
+
+
+def to_frame(a, rotated):
+    """Original-frame array (..., H, W) -> sweep frame (..., Hs, Ws)."""
+    return np.ascontiguousarray(np.swapaxes(a, -1, -2)) if rotated else np.ascontiguousarray(a)
+
+
+# Margins of the near-tie protocol: the GPU's NCC agrees with the oracle to
+# < 5e-6 and its view CDF (FP64 messages on FP32 costs) to ~1e-6, so a
+# decision whose oracle margin exceeds 2e-5 must come out the same.
+VTOL = 2e-5
+CTOL = 2e-5
+
+
+def fragile_mask(trace, vtol=VTOL, ctol=CTOL):
+    return (trace["view_margin"] < vtol) | (trace["cost_margin"] < ctol) | (trace["near_clamp"] > 0)
+
+
+def compare_sweep(gpu_depth_s, ora_depth_s, gpu_ids_s, ora_ids_s, trace, down,
+                  vtol=VTOL, ctol=CTOL, rtol=2e-5):
+    """Compares one sweep per column in processing order up to the first row
+    whose oracle decision came within the margin of flipping.  Returns a dict
+    of counts and the mask of clean columns."""
+    Hs, Ws = ora_depth_s.shape
+    order = np.arange(1, Hs) if down else np.arange(Hs - 2, -1, -1)
+    frag = fragile_mask(trace, vtol, ctol)[order]            # rows in processing order
+    before = np.cumsum(frag, axis=0) == 0                     # strictly before first fragile row
+    g = gpu_depth_s[order].astype(np.float64)
+    o = ora_depth_s[order]
+    dep_ok = np.abs(g - o) <= rtol * np.abs(o)
+    ids_ok = np.ones_like(dep_ok)
+    if gpu_ids_s is not None:
+        ids_ok = np.all(gpu_ids_s[:, order] == ora_ids_s[:, order], axis=0)
+    ok = dep_ok & ids_ok
+    compared = int(before.sum())
+    bad = before & ~ok
+    clean_cols = before.all(axis=0)
+    return dict(compared=compared, total=int(before.size), mismatches=int(bad.sum()),
+                depth_mismatch=int((before & ~dep_ok).sum()),
+                ids_mismatch=int((before & ~ids_ok).sum()), clean_cols=clean_cols,
+                first_bad=np.argwhere(bad)[:5].tolist())
+
+
+def rng_column_states(states, rotated, Ws):
+    return states[:Ws, 0] if rotated else states[0, :Ws]
+
+
+def within(a, b, rel=0.01):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.mean(np.abs(a - b) <= rel * np.abs(b)))
+
+
+TINY = configs.tiny()
