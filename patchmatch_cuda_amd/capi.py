@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpmx.so")
+# PMX_LIBRARY selects another build of the same C ABI (e.g. an instrumented one).
+LIB_PATH = os.environ.get("PMX_LIBRARY") or os.path.join(_HERE, "libpmx.so")
 
 PMX_OK = 0
 ERR_NAMES = {1: "PMX_ERR_ARG", 2: "PMX_ERR_HIP", 3: "PMX_ERR_ALLOC", 4: "PMX_ERR_STATE",

@@ -152,6 +152,8 @@ struct Timer {
 PmxSweepArgs sweep_args(pmx_engine* e, int rotated) {
     PmxSweepArgs a{};
     a.src = e->src + (size_t)PMX_SRC_PAD * e->src_pitch + PMX_SRC_PAD;
+    a.src_block = e->src;
+    a.src_bytes = (long)sizeof(float) * e->src_layer * e->V;
     a.src_layer = e->src_layer;
     a.src_pitch = e->src_pitch;
     a.srcW = e->maxW;
@@ -602,6 +604,12 @@ int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format, 
     e->src_pitch = round_up(maxW + 2 * PMX_SRC_PAD, 64);
     e->src_layer = (long)e->src_pitch * (maxH + 2 * PMX_SRC_PAD);
     const size_t src_bytes = sizeof(float) * (size_t)e->src_layer * V;
+    if (src_bytes >= (size_t)1 << 31) {
+        engine_free(e);
+        return fail(ctx, PMX_ERR_ARG,
+                    "source block of %zu bytes exceeds the 2 GiB addressable by one engine",
+                    src_bytes);
+    }
     ALLOC(e->src_raw, src_bytes);
     ALLOC(e->src, src_bytes);
     HIPCHK_E(hipMemsetAsync(e->src_raw, 0, src_bytes, s));

@@ -41,6 +41,8 @@ struct pmx_ctx {
 /* Arguments of the sweep and cost kernels (device pointers). */
 struct PmxSweepArgs {
     const float* src;      /* view 0, texel (0,0) of the padded source block */
+    const float* src_block; /* start of the padded source block */
+    long src_bytes;        /* its size (< 2^31: byte offsets are 32-bit) */
     long src_layer;        /* floats between layers */
     int src_pitch;         /* floats between rows */
     int srcW, srcH;        /* packed source size */

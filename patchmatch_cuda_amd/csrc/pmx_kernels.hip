@@ -356,17 +356,63 @@ struct PixelCtx {
     double scale;        /* n . K^-1 (x + 0.5, y + 0.5, 1) */
 };
 
-/* Projected window of one hypothesis, relative to the projected centre. */
-struct Proj {
-    float xcf, ycf;      /* centre in source texel space */
-    int ixc, iyc;        /* floor of the centre */
-    float fxc, fyc;      /* fractional part of the centre */
-    float ax, ay, bx, by, cx, cy;
+/* The padded source block as one raw buffer resource: 32-bit byte offsets,
+ * descriptor in SGPRs. */
+struct SrcBuf {
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t layer_bytes, pitch_bytes;
+    int srcW, srcH;
 };
 
-__device__ __forceinline__ Proj make_proj(const double* __restrict__ Hv, float depth,
-                                          const PixelCtx& pc) {
-    /* T = H1 - H2 / depth / scale (src/patchMatch.cu:465-473), in FP64. */
+__device__ __forceinline__ SrcBuf src_buf(const PmxSweepArgs& a) {
+    SrcBuf b;
+    b.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.src_block, (short)0, (int)a.src_bytes,
+                                               0x00020000);
+    b.layer_bytes = (uint32_t)a.src_layer * 4u;
+    b.pitch_bytes = (uint32_t)a.src_pitch * 4u;
+    b.srcW = a.srcW;
+    b.srcH = a.srcH;
+    return b;
+}
+
+/*
+ * One (view, depth) hypothesis of one pixel.  The plane-induced homography
+ * T = H1 - H2 / depth / scale (src/patchMatch.cu:465-473) is formed in FP64
+ * and the window is projected relative to its centre c' = T (x+0.5, y+0.5):
+ * a sample at offset (dx, dy) lands at
+ *     c' + ((ux dx + uy dy), (vx dx + vy dy)) / (1 + cx dx + cy dy),
+ * so FP32 only carries offsets of a few pixels.
+ */
+struct Hyp {
+    float ux, uy, vx, vy, cx, cy;
+    float fxc, fyc;              /* fractional part of the centre */
+    float lox, hix, loy, hiy;    /* relative positions that keep a tap inside the padded block */
+    uint32_t base;               /* byte offset of texel floor(c') */
+    float K;                     /* source value at the centre */
+};
+
+/* Bilinear tap with zero border (layered texture with border addressing and
+ * linear filtering, src/patchMatch.cu:192-195): position relative to the
+ * centre texel, clamped into the 2-pixel zero margin of the block. */
+__device__ __forceinline__ float tap(const SrcBuf& sb, const Hyp& q, float sx, float sy) {
+    sx = __builtin_amdgcn_fmed3f(sx, q.lox, q.hix);
+    sy = __builtin_amdgcn_fmed3f(sy, q.loy, q.hiy);
+    const int ix = (int)floorf(sx), iy = (int)floorf(sy);
+    const float al = __builtin_amdgcn_fractf(sx), be = __builtin_amdgcn_fractf(sy);
+    /* |iy| < 2^17 and the pitch < 2^17: a 24-bit multiply is exact */
+    const uint32_t off = (uint32_t)__mul24(iy, (int)sb.pitch_bytes) + ((uint32_t)ix << 2) + q.base;
+    const float2 t0 =
+        __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, 0, 0));
+    const float2 t1 = __builtin_bit_cast(
+        float2, __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, sb.pitch_bytes, 0));
+    const float top = fmaf(al, t0.y - t0.x, t0.x);
+    const float bot = fmaf(al, t1.y - t1.x, t1.x);
+    return fmaf(be, bot - top, top);
+}
+
+__device__ __forceinline__ Hyp make_hyp(const PmxSweepArgs& a, const SrcBuf& sb, int view,
+                                        float depth, const PixelCtx& pc) {
+    const double* __restrict__ Hv = a.Hm + (size_t)view * 18;
     const double inv = 1.0 / ((double)depth * pc.scale);
     double h[9];
 #pragma unroll
@@ -375,84 +421,128 @@ __device__ __forceinline__ Proj make_proj(const double* __restrict__ Hv, float d
     const double Y0 = fma(h[3], pc.cx, fma(h[4], pc.cy, h[5]));
     const double Z0 = fma(h[6], pc.cx, fma(h[7], pc.cy, h[8]));
     const double iz = 1.0 / Z0;
-    double xc = X0 * iz, yc = Y0 * iz;
-    xc = fmin(fmax(xc, -1.0e7), 1.0e7);
-    yc = fmin(fmax(yc, -1.0e7), 1.0e7);
-    const double flx = floor(xc), fly = floor(yc);
-    Proj p;
-    p.xcf = (float)xc;
-    p.ycf = (float)yc;
-    p.ixc = (int)flx;
-    p.iyc = (int)fly;
-    p.fxc = (float)(xc - flx);
-    p.fyc = (float)(yc - fly);
-    p.ax = (float)(h[0] * iz);
-    p.ay = (float)(h[1] * iz);
-    p.bx = (float)(h[3] * iz);
-    p.by = (float)(h[4] * iz);
-    p.cx = (float)(h[6] * iz);
-    p.cy = (float)(h[7] * iz);
-    return p;
+    const double xc = X0 * iz, yc = Y0 * iz;
+    /* centre texel, limited so that every byte offset fits 32 bits; a window
+     * whose centre is that far out lies entirely in the zero border */
+    const double flx = floor(fmin(fmax(xc, -65536.0), 65536.0));
+    const double fly = floor(fmin(fmax(yc, -65536.0), 65536.0));
+    const int ixc = (int)flx, iyc = (int)fly;
+    Hyp q;
+    q.fxc = (float)(xc - flx);
+    q.fyc = (float)(yc - fly);
+    q.lox = (float)(-PMX_SRC_PAD - ixc);
+    q.hix = (float)(sb.srcW - ixc);
+    q.loy = (float)(-PMX_SRC_PAD - iyc);
+    q.hiy = (float)(sb.srcH - iyc);
+    q.ux = (float)((h[0] - xc * h[6]) * iz);
+    q.uy = (float)((h[1] - xc * h[7]) * iz);
+    q.vx = (float)((h[3] - yc * h[6]) * iz);
+    q.vy = (float)((h[4] - yc * h[7]) * iz);
+    q.cx = (float)(h[6] * iz);
+    q.cy = (float)(h[7] * iz);
+    q.base = (uint32_t)view * sb.layer_bytes + (uint32_t)(iyc + PMX_SRC_PAD) * sb.pitch_bytes +
+             (uint32_t)(ixc + PMX_SRC_PAD) * 4u;
+    q.K = tap(sb, q, q.fxc, q.fyc);
+    return q;
+}
+
+/* Window rows [r0, r0 + nr) of one lane's window column; R is the static
+ * trip count (0: dynamic).  Sums are taken around K: A = sum e,
+ * B = sum e^2, C = sum e R' with e = I' - K.
+ * With a static trip count the rows are unrolled without branches, so all
+ * taps of the column can be in flight at once: rows i >= nr still fetch a
+ * (clamped, in-bounds) tap and read a zero padding row of the LDS window,
+ * and their e is selected to 0. */
+template <int R>
+__device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q,
+                                           const float* __restrict__ win, int Wn, float dxf,
+                                           float dy0, const float* __restrict__ wp, int nr,
+                                           float& A, float& B, float& C) {
+    const float nxa = fmaf(q.uy, dy0, q.ux * dxf);
+    const float nya = fmaf(q.vy, dy0, q.vx * dxf);
+    const float dza = fmaf(q.cy, dy0, fmaf(q.cx, dxf, 1.0f));
+    auto sample = [&](int i) -> float {
+        const float fi = (float)i;
+        const float rz = __builtin_amdgcn_rcpf(fmaf(q.cy, fi, dza));
+        return tap(sb, q, fmaf(fmaf(q.uy, fi, nxa), rz, q.fxc),
+                   fmaf(fmaf(q.vy, fi, nya), rz, q.fyc));
+    };
+    if constexpr (R > 0) {
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const float val = sample(i);
+            const float e = (i < nr) ? val - q.K : 0.0f;
+            A += e;
+            B = fmaf(e, e, B);
+            C = fmaf(e, wp[i * Wn], C);
+        }
+    } else {
+#pragma unroll 4
+        for (int i = 0; i < nr; i++) {
+            const float e = sample(i) - q.K;
+            A += e;
+            B = fmaf(e, e, B);
+            C = fmaf(e, wp[i * Wn], C);
+        }
+    }
+}
+
+/* 1 - NCC from the centred sums (src/patchMatch.cu:594-610), final
+ * normalisation in FP64:  n var' = B - A^2/n,
+ * sum I' (Rq - mean) = C + K D  (D = sum (Rq - mean), 0 unless the
+ * tile-index shift applies). */
+__device__ __forceinline__ float ncc_finish(float A, float B, float C, float K,
+                                            const PixelCtx& pc) {
+    /* sum I'^2 == 0 <=> every in-bounds sample is 0; the centre sample K is
+     * one of them */
+    if (K == 0.0f && B == 0.0f) return kSentinel;
+    const double n = (double)pc.npx;
+    double c2 = (double)B - (double)A * (double)A / n;
+    c2 = c2 < 0.00001 ? 0.0 : c2;
+    const double den = sqrt((double)pc.c1 * c2);
+    if (den == 0.0) return 1.0f;
+    return (float)(1.0 - ((double)C + (double)K * (double)pc.D) / den);
 }
 
 /*
- * One hypothesis per G-lane group: returns 1 - NCC in every lane of the
- * group (src/patchMatch.cu:433-611).  `win` holds R' = Rq - mean for the
- * window (row-major, Wn x Wn), Rq being the reference value the reference
- * pairs with each sample (tile-index quirk already applied).
+ * Evaluates nt <= 64/G hypotheses (tview[t], tdep[t]) of one pixel and
+ * writes 1 - NCC to tout[t].  One G-lane group per hypothesis, lanes over
+ * window columns; when fewer hypotheses than groups remain, each is split
+ * over 2 or 4 groups by window rows so no group idles.
  */
-template <int G>
-__device__ __forceinline__ float ncc_group(const PmxSweepArgs& a, const float* __restrict__ win,
-                                           int Wn, int gl, bool active, int view, float depth,
-                                           const PixelCtx& pc) {
-    float A = 0.0f, B = 0.0f, C = 0.0f, M = 0.0f;
-    /* Source samples are accumulated around K, the sample at the projected
-     * window centre, so B - A^2/n does not cancel when the source window's
-     * mean is far from the reference's (wrong-depth hypotheses). */
-    float K = 0.0f;
-    const float* __restrict__ base = a.src + (size_t)view * a.src_layer;
-    const int pitch = a.src_pitch;
-    const int maxx = a.srcW, maxy = a.srcH;
-    Proj pj{};
+template <int G, int WN>
+__device__ __forceinline__ void eval_pass(const PmxSweepArgs& a, const SrcBuf& sb,
+                                          const float* __restrict__ win, int lane, int nt,
+                                          const int* tview, const float* tdep, float* tout,
+                                          const PixelCtx& pc) {
+    constexpr int NG = kLanes / G;
+    const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
+    const int grp = lane / G, gl = lane % G;
+    const int k = (NG >= 4 && nt == 1) ? 4 : ((NG >= 2 && nt <= 2) ? 2 : 1);
+    const int t = grp / k, sub = grp - t * k;
+    const bool active = t < nt;
+    float A = 0.0f, B = 0.0f, C = 0.0f, K = 0.0f;
     if (active) {
-        pj = make_proj(a.Hm + (size_t)view * 18, depth, pc);
-        const int gx = min(max(pj.ixc, -PMX_SRC_PAD), maxx);
-        const int gy = min(max(pj.iyc, -PMX_SRC_PAD), maxy);
-        const float* __restrict__ p = base + (ptrdiff_t)gy * pitch + gx;
-        const float t00 = p[0], t10 = p[1], t01 = p[pitch], t11 = p[pitch + 1];
-        const float top = fmaf(pj.fxc, t10 - t00, t00);
-        const float bot = fmaf(pj.fxc, t11 - t01, t01);
-        K = fmaf(pj.fyc, bot - top, top);
-    }
-    if (active && pc.lane_in) {
-        const float dxf = (float)(gl - a.hw);
-        const float axd = pj.ax * dxf, bxd = pj.bx * dxf, cxd = pj.cx * dxf;
-#pragma unroll 3
-        for (int dyi = pc.dy_lo; dyi <= pc.dy_hi; dyi++) {
-            const float dyf = (float)(dyi - a.hw);
-            const float dX = fmaf(pj.ay, dyf, axd);
-            const float dY = fmaf(pj.by, dyf, bxd);
-            const float dZ = fmaf(pj.cy, dyf, cxd);
-            const float rz = __builtin_amdgcn_rcpf(1.0f + dZ);
-            float sx = pj.fxc + fmaf(-pj.xcf, dZ, dX) * rz;
-            float sy = pj.fyc + fmaf(-pj.ycf, dZ, dY) * rz;
-            sx = fminf(fmaxf(sx, -1.0e7f), 1.0e7f);
-            sy = fminf(fmaxf(sy, -1.0e7f), 1.0e7f);
-            const float fx = floorf(sx), fy = floorf(sy);
-            const float al = sx - fx, be = sy - fy;
-            const int gx = min(max(pj.ixc + (int)fx, -PMX_SRC_PAD), maxx);
-            const int gy = min(max(pj.iyc + (int)fy, -PMX_SRC_PAD), maxy);
-            const float* __restrict__ p = base + (ptrdiff_t)gy * pitch + gx;
-            const float t00 = p[0], t10 = p[1], t01 = p[pitch], t11 = p[pitch + 1];
-            const float top = fmaf(al, t10 - t00, t00);
-            const float bot = fmaf(al, t11 - t01, t01);
-            const float val = fmaf(be, bot - top, top);
-            const float e = val - K;
-            const float rv = win[dyi * Wn + gl];
-            A += e;
-            B = fmaf(e, e, B);
-            C = fmaf(e, rv, C);
-            M = fmaxf(M, fabsf(val));
+        const Hyp q = make_hyp(a, sb, tview[t], tdep[t], pc);
+        K = q.K;
+        const int R = (Wn + k - 1) / k;
+        const int r0 = max(sub * R, pc.dy_lo);
+        const int r1 = min(min((sub + 1) * R, Wn), pc.dy_hi + 1);
+        const int nr = r1 - r0;
+        if (pc.lane_in && nr > 0) {
+            const float dxf = (float)(gl - a.hw);
+            const float dy0 = (float)(r0 - a.hw);
+            const float* wp = win + r0 * Wn + gl;
+            if constexpr (WN > 0) {
+                if (k == 1)
+                    accum_rows<WN>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C);
+                else if (k == 2)
+                    accum_rows<(WN + 1) / 2>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C);
+                else
+                    accum_rows<(WN + 3) / 4>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C);
+            } else {
+                accum_rows<0>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C);
+            }
         }
     }
 #pragma unroll
@@ -460,18 +550,13 @@ __device__ __forceinline__ float ncc_group(const PmxSweepArgs& a, const float* _
         A += __shfl_xor(A, m, G);
         B += __shfl_xor(B, m, G);
         C += __shfl_xor(C, m, G);
-        M = fmaxf(M, __shfl_xor(M, m, G));
     }
-    /* sum I'^2 == 0 <=> every in-bounds sample is 0 (src/patchMatch.cu:594-595) */
-    if (!(M > 0.0f)) return kSentinel;
-    /* Final normalisation in FP64 (src/patchMatch.cu:597-610):
-     *   n var' = B - A^2/n,  sum I' (Rq - mean) = C + K * D. */
-    const double n = (double)pc.npx;
-    double c2 = (double)B - (double)A * (double)A / n;
-    c2 = c2 < 0.00001 ? 0.0 : c2;
-    const double den = sqrt((double)pc.c1 * c2);
-    if (den == 0.0) return 1.0f;
-    return (float)(1.0 - ((double)C + (double)K * (double)pc.D) / den);
+    for (int m = G; m < G * k; m <<= 1) {
+        A += __shfl_xor(A, m, kLanes);
+        B += __shfl_xor(B, m, kLanes);
+        C += __shfl_xor(C, m, kLanes);
+    }
+    if (active && sub == 0 && gl == 0) tout[t] = ncc_finish(A, B, C, K, pc);
 }
 
 /* Loads the reference window of pixel (x, y) for one lane's share:
@@ -500,11 +585,13 @@ __device__ __forceinline__ void load_window(const PmxSweepArgs& a, int x, int y,
     }
 }
 
-/* Writes a loaded window as R' = Rq - mean into LDS; returns the wave sum of
- * R' over the in-bounds positions. */
+/* Writes a loaded window as R' = Rq - mean into LDS.  With need_d (the
+ * tile-index shift applies) returns the wave sum D of R' over the in-bounds
+ * positions; otherwise D is exactly 0 (the in-bounds values sum to the box
+ * sum) and 0 is returned. */
 template <int PF>
 __device__ __forceinline__ float store_window(float* __restrict__ win, const float* vals, float mu,
-                                              int Wn, int lane) {
+                                              int Wn, int lane, bool need_d) {
     float d = 0.0f;
 #pragma unroll
     for (int k = 0; k < PF; k++) {
@@ -517,6 +604,7 @@ __device__ __forceinline__ float store_window(float* __restrict__ win, const flo
             d += r;
         }
     }
+    if (!need_d) return 0.0f;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) d += __shfl_xor(d, m, kLanes);
     return d;
@@ -542,16 +630,22 @@ __device__ __forceinline__ void pixel_ctx(const PmxSweepArgs& a, int x, int y, i
 /* ------------------------------------------------------ cost all views */
 /* computeAllCostGivenDepth (src/patchMatch.cu:775-816): one wavefront per
  * pixel, G-lane groups over views.  Writes the row-major state. */
-template <int G, int PF>
+template <int G, int PF, int WN>
 __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
-    extern __shared__ float win[];
+    extern __shared__ float smem[];
+    const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
+    float* win = smem;                         /* Wn rows + Wn zero rows, x Wn */
+    int* tview = (int*)(smem + 2 * Wn * Wn);   /* 64 */
+    float* tdep = (float*)(tview + 64);        /* 64 */
+    float* tout = tdep + 64;                   /* 64 */
     const int lane = threadIdx.x;
-    const int gl = lane % G, grp = lane / G;
+    for (int i = Wn * Wn + (int)threadIdx.x; i < 2 * Wn * Wn; i += kLanes) win[i] = 0.0f;
+    const int gl = lane % G;
     constexpr int NG = kLanes / G;
     const long pid = (long)blockIdx.x;
     const int x = (int)(pid % a.W), y = (int)(pid / a.W);
-    const int Wn = 2 * a.hw + 1;
     const float inv_wn = 1.0f / (float)Wn;
+    const SrcBuf sb = src_buf(a);
     PixelCtx pc;
     pixel_ctx(a, x, y, gl, pc);
     const float2 st = a.stats[(size_t)y * a.stats_pitch + x];
@@ -563,20 +657,19 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
     const int qy = a.rotated ? max(0, a.hw - y) : 0;
     float vals[PF];
     load_window<PF>(a, x, y, qx, qy, Wn, inv_wn, lane, vals);
-    pc.D = store_window<PF>(win, vals, pc.mu, Wn, lane);
-    /* Without the tile-index shift the in-bounds reference values sum to the
-     * box sum exactly, so D = sum (Rq - mean) is 0; use the exact value. */
-    if (qx == 0 && qy == 0) pc.D = 0.0f;
-    __syncthreads();
+    pc.D = store_window<PF>(win, vals, pc.mu, Wn, lane, qx != 0 || qy != 0);
     /* state is row-major: column c = y, row r = x */
     const float depth = a.depth[(size_t)y * a.depth_ld + x];
-    float* cst = a.cost + (size_t)y * a.cost_ld + (size_t)x * a.V;
-    for (int v0 = 0; v0 < a.V; v0 += NG) {
-        const int v = v0 + grp;
-        const bool act = v < a.V;
-        const float c = ncc_group<G>(a, win, Wn, gl, act, act ? v : 0, depth, pc);
-        if (act && gl == 0) cst[v] = c;
+    if (lane < a.V) {
+        tview[lane] = lane;
+        tdep[lane] = depth;
     }
+    __syncthreads();
+    for (int v0 = 0; v0 < a.V; v0 += NG)
+        eval_pass<G, WN>(a, sb, win, lane, min(NG, a.V - v0), tview + v0, tdep + v0, tout + v0, pc);
+    __syncthreads();
+    float* cst = a.cost + (size_t)y * a.cost_ld + (size_t)x * a.V;
+    if (lane < a.V) cst[lane] = tout[lane];
     if (lane == 0 && a.counters) {
         const int nx = pc.npx;
         atomicAdd(&a.counters[2], (unsigned long long)a.V);
@@ -647,26 +740,29 @@ __device__ __forceinline__ double msg_forward(double m, double e, const MsgConst
  * is reused when the view was sampled (the reference recomputes the same
  * value).
  */
-template <int G, int PF>
+template <int G, int PF, int WN>
 __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
     extern __shared__ float smem[];
-    const int Wn = 2 * a.hw + 1;
-    const int WW = Wn * Wn;
+    const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
+    /* two window buffers, each Wn rows followed by Wn zero rows */
+    const int WW = 2 * Wn * Wn;
     float* winbuf = smem;                 /* 2 x WW */
     float* tres = smem + 2 * WW;          /* task results, 2*64 */
+    for (int i = (int)threadIdx.x; i < 2 * WW; i += kLanes) winbuf[i] = 0.0f;
     int* tview = (int*)(tres + 128);      /* task views, 128 */
     float* tdep = (float*)(tview + 128);  /* task depths, 128 */
     int* vj = (int*)(tdep + 128);         /* sampled views, S */
 
     constexpr int NG = kLanes / G;
     const int lane = threadIdx.x;
-    const int gl = lane % G, grp = lane / G;
+    const int gl = lane % G;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= a.Ws) return;
     const int V = a.V, Hs = a.Hs;
     const bool is_view = lane < V;
     const int vl = is_view ? lane : 0;
     const float inv_wn = 1.0f / (float)Wn;
+    const SrcBuf sb = src_buf(a);
 
     const MsgConst mc = msg_const(a.alpha2);
 
@@ -718,6 +814,7 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
     };
     const int qx_col = a.rotated ? 0 : max(0, a.hw - c);
     const int qy_col = a.rotated ? max(0, a.hw - c) : 0;
+    const bool need_d = qx_col != 0 || qy_col != 0;
 
     /* prefetch row r0 + step */
     int r = r0 + step;
@@ -737,7 +834,7 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
     if (Hs > 1) prefetch(r);
     int buf = 0;
     float D = 0.0f;
-    if (Hs > 1) D = store_window<PF>(winbuf, pf_win, pf_stat.x, Wn, lane);
+    if (Hs > 1) D = store_window<PF>(winbuf, pf_win, pf_stat.x, Wn, lane, need_d);
 
     unsigned long long n_evals = 0, n_samples = 0;
 
@@ -755,9 +852,7 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
         pixel_ctx(a, x, y, gl, pc);
         pc.mu = st.x;
         pc.c1 = st.y;
-        /* D = sum (Rq - mean) is exactly 0 unless the tile-index shift
-         * applies to this column */
-        pc.D = (qx_col == 0 && qy_col == 0) ? 0.0f : D;
+        pc.D = D;
         const int npx = pc.npx;
 
         /* forward message with the stored costs, posterior, CDF
@@ -807,14 +902,9 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
 
         /* phase B: former and random depth on the sampled views */
         const int nt = 2 * U;
-        for (int t0 = 0; t0 < nt; t0 += NG) {
-            const int t = t0 + grp;
-            const bool act = t < nt;
-            const int tv = act ? tview[t] : 0;
-            const float td = act ? tdep[t] : 1.0f;
-            const float cc = ncc_group<G>(a, win, Wn, gl, act, tv, td, pc);
-            if (act && gl == 0) tres[t] = cc;
-        }
+        for (int t0 = 0; t0 < nt; t0 += NG)
+            eval_pass<G, WN>(a, sb, win, lane, min(NG, nt - t0), tview + t0, tdep + t0, tres + t0,
+                             pc);
         __syncthreads();
         float nf = 0.0f, nr = 0.0f;
         if (in_sel) {
@@ -853,13 +943,9 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
                 tdep[rrank] = best;
             }
             __syncthreads();
-            for (int t0 = 0; t0 < nrest; t0 += NG) {
-                const int t = t0 + grp;
-                const bool act = t < nrest;
-                const int tv = act ? tview[t] : 0;
-                const float cc = ncc_group<G>(a, win, Wn, gl, act, tv, best, pc);
-                if (act && gl == 0) tres[64 + t] = cc;
-            }
+            for (int t0 = 0; t0 < nrest; t0 += NG)
+                eval_pass<G, WN>(a, sb, win, lane, min(NG, nrest - t0), tview + t0, tdep + t0,
+                                 tres + 64 + t0, pc);
             __syncthreads();
             if (in_sel) cnew = (idx == 0) ? nf : nr;
             if (in_rest) cnew = tres[64 + rrank];
@@ -876,7 +962,7 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
         /* next window into the other buffer */
         if (k + 1 < Hs) {
             buf ^= 1;
-            D = store_window<PF>(winbuf + buf * WW, pf_win, pf_stat.x, Wn, lane);
+            D = store_window<PF>(winbuf + buf * WW, pf_win, pf_stat.x, Wn, lane, need_d);
         }
     }
     if (lane == 0) {
@@ -984,34 +1070,41 @@ hipError_t pmx_launch_scatter_costs(hipStream_t s, const float* in, int in_ld, f
 }
 
 namespace {
-template <int G, int PF>
+template <int G, int PF, int WN>
 hipError_t launch_cost_all_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
-    const size_t sh = sizeof(float) * Wn * Wn;
-    hipLaunchKernelGGL((pmx_cost_all_kernel<G, PF>), dim3((unsigned)((long)a.W * a.H)), dim3(64),
+    const size_t sh = sizeof(float) * (2 * Wn * Wn + 3 * 64);
+    hipLaunchKernelGGL((pmx_cost_all_kernel<G, PF, WN>), dim3((unsigned)((long)a.W * a.H)), dim3(64),
                        sh, s, a);
     return hipGetLastError();
 }
 
-template <int G, int PF>
+template <int G, int PF, int WN>
 hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
-    const size_t sh = sizeof(float) * (2 * Wn * Wn + 128 + 128 + 128 + 64);
-    hipLaunchKernelGGL((pmx_sweep_kernel<G, PF>), dim3(a.Ws), dim3(64), sh, s, a);
+    const size_t sh = sizeof(float) * (4 * Wn * Wn + 128 + 128 + 128 + 64);
+    hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN>), dim3(a.Ws), dim3(64), sh, s, a);
     return hipGetLastError();
 }
 } // namespace
 
+/* Window sizes 11 and 15 (the configured workloads) get fully unrolled
+ * row loops; every other size runs the same code with a dynamic trip
+ * count. */
 hipError_t pmx_launch_cost_all(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
-    if (Wn <= 16) return launch_cost_all_t<16, 4>(s, a);
-    if (Wn <= 32) return launch_cost_all_t<32, 16>(s, a);
-    return launch_cost_all_t<64, 44>(s, a);
+    if (Wn == 15) return launch_cost_all_t<16, 4, 15>(s, a);
+    if (Wn == 11) return launch_cost_all_t<16, 2, 11>(s, a);
+    if (Wn <= 16) return launch_cost_all_t<16, 4, 0>(s, a);
+    if (Wn <= 32) return launch_cost_all_t<32, 16, 0>(s, a);
+    return launch_cost_all_t<64, 44, 0>(s, a);
 }
 
 hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
-    if (Wn <= 16) return launch_sweep_t<16, 4>(s, a);
-    if (Wn <= 32) return launch_sweep_t<32, 16>(s, a);
-    return launch_sweep_t<64, 44>(s, a);
+    if (Wn == 15) return launch_sweep_t<16, 4, 15>(s, a);
+    if (Wn == 11) return launch_sweep_t<16, 2, 11>(s, a);
+    if (Wn <= 16) return launch_sweep_t<16, 4, 0>(s, a);
+    if (Wn <= 32) return launch_sweep_t<32, 16, 0>(s, a);
+    return launch_sweep_t<64, 44, 0>(s, a);
 }
