@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads (default: min(16, cpu count))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-waves", type=int, default=0,
+                    help="wavefronts per sweep column (1, 2, 4; 0 = engine default)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -150,6 +152,8 @@ def main():
         f"{time.perf_counter() - t0:.1f} s")
     ctx = capi.Context(0)
     eng = capi.Engine(ctx, rp, sc.ref, list(sc.srcs), sc.cams)
+    if args.sweep_waves:
+        eng.set_sweep_waves(args.sweep_waves)
     wall, st = run_gpu_steps(eng, args.steps, args.warmup)
     npx = spec.W * spec.H
     ms_step = wall * 1e3 / args.steps
@@ -180,6 +184,7 @@ def main():
             "iterations": rp.num_iterations,
             "num_samples": rp.num_samples,
             "method": "plain",
+            "sweep_waves": eng.sweep_waves(),
         },
         "roofline": {
             "bound": "mfma",

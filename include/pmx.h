@@ -203,6 +203,11 @@ int pmx_engine_export(pmx_engine* eng, float** depth, int* depth_pitch,
                       float** costs, int* cost_pitch, uint8_t** used_ids,
                       int* ids_pitch);
 int pmx_engine_set_timing(pmx_engine* eng, int enabled);
+/* Launch shape of the sweep kernel: wavefronts per sweep column (1, 2 or
+ * 4).  No reference counterpart (its sweeps use one 32-thread block per 32
+ * columns, src/patchMatch.cu:334); chosen by the autotuner. */
+int pmx_engine_set_sweep_waves(pmx_engine* eng, int waves);
+int pmx_engine_get_sweep_waves(pmx_engine* eng, int* waves);
 int pmx_engine_get_stats(pmx_engine* eng, pmx_stats* out);
 int pmx_engine_reset_stats(pmx_engine* eng);
 

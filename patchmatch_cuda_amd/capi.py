@@ -116,6 +116,8 @@ SIGNATURES = {
     "pmx_engine_export": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i), ctypes.POINTER(_vp),
                           ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_i)],
     "pmx_engine_set_timing": [_vp, _i],
+    "pmx_engine_set_sweep_waves": [_vp, _i],
+    "pmx_engine_get_sweep_waves": [_vp, ctypes.POINTER(_i)],
     "pmx_engine_get_stats": [_vp, ctypes.POINTER(Stats)],
     "pmx_engine_reset_stats": [_vp],
 }
@@ -298,6 +300,14 @@ class Engine:
 
     def set_timing(self, on=True):
         self._c(lib().pmx_engine_set_timing(self.h, int(on)))
+
+    def set_sweep_waves(self, n):
+        self._c(lib().pmx_engine_set_sweep_waves(self.h, int(n)))
+
+    def sweep_waves(self):
+        n = ctypes.c_int()
+        self._c(lib().pmx_engine_get_sweep_waves(self.h, ctypes.byref(n)))
+        return n.value
 
     def stats(self):
         s = Stats()

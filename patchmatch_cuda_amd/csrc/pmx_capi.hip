@@ -228,6 +228,16 @@ int to_layout(pmx_engine* e, int cm, Timer* tm, bool with_costs = true) {
     return PMX_OK;
 }
 
+/* In-bounds window positions summed over the image: sum_x nx(x) * sum_y ny(y). */
+static unsigned long long window_samples(int W, int H, int hw) {
+    auto span = [hw](int n) {
+        unsigned long long s = 0;
+        for (int i = 0; i < n; i++) s += (unsigned long long)(std::min(i + hw, n - 1) - std::max(i - hw, 0) + 1);
+        return s;
+    };
+    return span(W) * span(H);
+}
+
 int do_cost_all(pmx_engine* e, int rotated, Timer* tm) {
     PMXCHK(to_layout(e, 0, tm, false));
     PmxSweepArgs a = sweep_args(e, 1);
@@ -235,6 +245,8 @@ int do_cost_all(pmx_engine* e, int rotated, Timer* tm) {
     const int id = tm ? tm->begin(K_COST_ALL) : -1;
     HIPCHK(e->ctx, pmx_launch_cost_all(e->ctx->stream, a));
     if (tm) tm->end(id);
+    e->cost_all_evals += (unsigned long long)e->W * e->H * e->V;
+    e->cost_all_samples += window_samples(e->W, e->H, e->p.half_window) * (unsigned long long)e->V;
     return PMX_OK;
 }
 
@@ -250,7 +262,7 @@ int do_sweep(pmx_engine* e, int down, int rotated, int S, bool ids, Timer* tm) {
         a.ids = e->used_ids;
     }
     const int id = tm ? tm->begin(K_SWEEP) : -1;
-    HIPCHK(e->ctx, pmx_launch_sweep(e->ctx->stream, a));
+    HIPCHK(e->ctx, pmx_launch_sweep(e->ctx->stream, a, e->sweep_waves));
     if (tm) tm->end(id);
     e->stats_acc.sweep_launches++;
     return PMX_OK;
@@ -970,6 +982,21 @@ int pmx_engine_set_timing(pmx_engine* e, int enabled) {
     return PMX_OK;
 }
 
+int pmx_engine_set_sweep_waves(pmx_engine* e, int waves) {
+    PMXCHK(check_engine(e));
+    if (waves != 1 && waves != 2 && waves != 4)
+        return fail(e->ctx, PMX_ERR_ARG, "sweep waves must be 1, 2 or 4 (got %d)", waves);
+    e->sweep_waves = waves;
+    return PMX_OK;
+}
+
+int pmx_engine_get_sweep_waves(pmx_engine* e, int* waves) {
+    PMXCHK(check_engine(e));
+    if (!waves) return fail(e->ctx, PMX_ERR_ARG, "waves is NULL");
+    *waves = e->sweep_waves;
+    return PMX_OK;
+}
+
 int pmx_engine_get_stats(pmx_engine* e, pmx_stats* out) {
     PMXCHK(check_engine(e));
     if (!out) return fail(e->ctx, PMX_ERR_ARG, "out is NULL");
@@ -979,8 +1006,8 @@ int pmx_engine_get_stats(pmx_engine* e, pmx_stats* out) {
     *out = e->stats_acc;
     out->ncc_evals = c[0];
     out->ncc_samples = c[1];
-    out->cost_all_evals = c[2];
-    out->cost_all_samples = c[3];
+    out->cost_all_evals = e->cost_all_evals;
+    out->cost_all_samples = e->cost_all_samples;
     return PMX_OK;
 }
 
@@ -989,6 +1016,8 @@ int pmx_engine_reset_stats(pmx_engine* e) {
     HIPCHK(e->ctx, hipMemsetAsync(e->counters, 0, sizeof(unsigned long long) * 4, e->ctx->stream));
     HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
     e->stats_acc = pmx_stats{};
+    e->cost_all_evals = 0;
+    e->cost_all_samples = 0;
     return PMX_OK;
 }
 

@@ -115,7 +115,10 @@ struct pmx_engine {
     float* out_cost = nullptr;  int out_cost_pitch = 0;
 
     unsigned long long* counters = nullptr;
+    unsigned long long cost_all_evals = 0;   /* closed form, counted on the host */
+    unsigned long long cost_all_samples = 0;
     bool prepared = false;
+    int sweep_waves = 2;        /* wavefronts per sweep column (1, 2 or 4) */
 
     /* timing */
     bool timing = false;
@@ -143,7 +146,7 @@ hipError_t pmx_launch_compose(hipStream_t s, const double* cams /* (V+1)*21 */, 
 hipError_t pmx_launch_convert_u8(hipStream_t s, const uint8_t* in, int in_pitch, float* out,
                                  int out_ld, int width, int height);
 hipError_t pmx_launch_cost_all(hipStream_t s, const PmxSweepArgs& a);
-hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a);
+hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a, int waves);
 hipError_t pmx_launch_gather_costs(hipStream_t s, const float* cost_rm, int cost_ld, float* out,
                                    int out_ld, int width, int height, int V);
 hipError_t pmx_launch_scatter_costs(hipStream_t s, const float* in, int in_ld, float* cost_rm,

@@ -410,17 +410,27 @@ __device__ __forceinline__ float tap(const SrcBuf& sb, const Hyp& q, float sx, f
     return fmaf(be, bot - top, top);
 }
 
-__device__ __forceinline__ Hyp make_hyp(const PmxSweepArgs& a, const SrcBuf& sb, int view,
+/* 1/x in FP64: hardware reciprocal refined by two Newton steps (full
+ * double precision for normal x). */
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
+/* Hm: the V x 18 homographies (H1, H2), a copy in LDS */
+__device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const SrcBuf& sb, int view,
                                         float depth, const PixelCtx& pc) {
-    const double* __restrict__ Hv = a.Hm + (size_t)view * 18;
-    const double inv = 1.0 / ((double)depth * pc.scale);
+    const double* __restrict__ Hv = Hm + view * 18;
+    const double inv = rcp64((double)depth * pc.scale);
     double h[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) h[k] = fma(-Hv[9 + k], inv, Hv[k]);
     const double X0 = fma(h[0], pc.cx, fma(h[1], pc.cy, h[2]));
     const double Y0 = fma(h[3], pc.cx, fma(h[4], pc.cy, h[5]));
     const double Z0 = fma(h[6], pc.cx, fma(h[7], pc.cy, h[8]));
-    const double iz = 1.0 / Z0;
+    const double iz = rcp64(Z0);
     const double xc = X0 * iz, yc = Y0 * iz;
     /* centre texel, limited so that every byte offset fits 32 bits; a window
      * whose centre is that far out lies entirely in the zero border */
@@ -505,25 +515,32 @@ __device__ __forceinline__ float ncc_finish(float A, float B, float C, float K,
 }
 
 /*
- * Evaluates nt <= 64/G hypotheses (tview[t], tdep[t]) of one pixel and
- * writes 1 - NCC to tout[t].  One G-lane group per hypothesis, lanes over
- * window columns; when fewer hypotheses than groups remain, each is split
- * over 2 or 4 groups by window rows so no group idles.
+ * One pass over nt hypotheses (tview[t], tdep[t]) of one pixel on the NW
+ * wavefronts of a workgroup (NW*64/G lane groups).  One group per
+ * hypothesis, lanes over window columns; with fewer hypotheses than groups
+ * each is split by window rows over k = 2, 4, ... groups so no group idles.
+ * If k <= groups per wave, the hypothesis stays inside one wave and its
+ * 1 - NCC is written to tout[t]; otherwise each wave writes its partial
+ * sums to part[t * NW + slice] and the caller combines them after a
+ * barrier (combine_pass).  Returns k.
  */
-template <int G, int WN>
-__device__ __forceinline__ void eval_pass(const PmxSweepArgs& a, const SrcBuf& sb,
-                                          const float* __restrict__ win, int lane, int nt,
-                                          const int* tview, const float* tdep, float* tout,
-                                          const PixelCtx& pc) {
+template <int G, int WN, int NW>
+__device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __restrict__ Hm,
+                                         const SrcBuf& sb, const float* __restrict__ win,
+                                         int wave, int lane, int nt, const int* tview,
+                                         const float* tdep, float* tout, float4* part,
+                                         const PixelCtx& pc) {
     constexpr int NG = kLanes / G;
+    constexpr int NGT = NG * NW;
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
-    const int grp = lane / G, gl = lane % G;
-    const int k = (NG >= 4 && nt == 1) ? 4 : ((NG >= 2 && nt <= 2) ? 2 : 1);
-    const int t = grp / k, sub = grp - t * k;
+    int k = 1;
+    while (2 * k * nt <= NGT) k *= 2;
+    const int gg = wave * NG + lane / G, gl = lane % G;
+    const int t = gg / k, sub = gg - t * k;
     const bool active = t < nt;
     float A = 0.0f, B = 0.0f, C = 0.0f, K = 0.0f;
     if (active) {
-        const Hyp q = make_hyp(a, sb, tview[t], tdep[t], pc);
+        const Hyp q = make_hyp(Hm, sb, tview[t], tdep[t], pc);
         K = q.K;
         const int R = (Wn + k - 1) / k;
         const int r0 = max(sub * R, pc.dy_lo);
@@ -534,12 +551,13 @@ __device__ __forceinline__ void eval_pass(const PmxSweepArgs& a, const SrcBuf& s
             const float dy0 = (float)(r0 - a.hw);
             const float* wp = win + r0 * Wn + gl;
             if constexpr (WN > 0) {
-                if (k == 1)
-                    accum_rows<WN>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C);
-                else if (k == 2)
-                    accum_rows<(WN + 1) / 2>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C);
-                else
-                    accum_rows<(WN + 3) / 4>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C);
+                switch (k) {
+                case 1: accum_rows<WN>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
+                case 2: accum_rows<(WN + 1) / 2>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
+                case 4: accum_rows<(WN + 3) / 4>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
+                case 8: accum_rows<(WN + 7) / 8>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
+                default: accum_rows<(WN + 15) / 16>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
+                }
             } else {
                 accum_rows<0>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C);
             }
@@ -551,12 +569,35 @@ __device__ __forceinline__ void eval_pass(const PmxSweepArgs& a, const SrcBuf& s
         B += __shfl_xor(B, m, G);
         C += __shfl_xor(C, m, G);
     }
-    for (int m = G; m < G * k; m <<= 1) {
+    const int kw = k < NG ? k : NG; /* groups of the hypothesis inside this wave */
+    for (int m = G; m < G * kw; m <<= 1) {
         A += __shfl_xor(A, m, kLanes);
         B += __shfl_xor(B, m, kLanes);
         C += __shfl_xor(C, m, kLanes);
     }
-    if (active && sub == 0 && gl == 0) tout[t] = ncc_finish(A, B, C, K, pc);
+    if (k <= NG) {
+        if (active && sub == 0 && gl == 0) tout[t] = ncc_finish(A, B, C, K, pc);
+    } else if (active && lane == 0) {
+        part[t * NW + sub / NG] = make_float4(A, B, C, K);
+    }
+    return k;
+}
+
+/* Second half of a pass whose hypotheses spanned several waves: every wave
+ * sums the per-wave partials in slice order and writes the same 1 - NCC. */
+template <int NW>
+__device__ __forceinline__ void combine_pass(int lane, int nt, int slices, const float4* part,
+                                             float* tout, const PixelCtx& pc) {
+    if (lane < nt) {
+        float4 s = part[lane * NW];
+        for (int w = 1; w < slices; w++) {
+            const float4 q = part[lane * NW + w];
+            s.x += q.x;
+            s.y += q.y;
+            s.z += q.z;
+        }
+        tout[lane] = ncc_finish(s.x, s.y, s.z, s.w, pc);
+    }
 }
 
 /* Loads the reference window of pixel (x, y) for one lane's share:
@@ -628,20 +669,27 @@ __device__ __forceinline__ void pixel_ctx(const PmxSweepArgs& a, int x, int y, i
 }
 
 /* ------------------------------------------------------ cost all views */
+/* Copies the V x 18 homographies into LDS (read by every hypothesis). */
+__device__ __forceinline__ void load_homographies(const PmxSweepArgs& a, double* Hs) {
+    for (int i = (int)threadIdx.x; i < a.V * 18; i += (int)blockDim.x) Hs[i] = a.Hm[i];
+}
+
 /* computeAllCostGivenDepth (src/patchMatch.cu:775-816): one wavefront per
  * pixel, G-lane groups over views.  Writes the row-major state. */
 template <int G, int PF, int WN>
 __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
-    extern __shared__ float smem[];
+    extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
-    float* win = smem;                         /* Wn rows + Wn zero rows, x Wn */
-    int* tview = (int*)(smem + 2 * Wn * Wn);   /* 64 */
-    float* tdep = (float*)(tview + 64);        /* 64 */
-    float* tout = tdep + 64;                   /* 64 */
+    double* Hs = smem_d;                               /* V x 18 */
+    float* win = (float*)(smem_d + a.V * 18);          /* Wn rows + Wn zero rows, x Wn */
+    int* tview = (int*)(win + 2 * Wn * Wn);            /* 64 */
+    float* tdep = (float*)(tview + 64);                /* 64 */
+    float* tout = tdep + 64;                           /* 64 */
     const int lane = threadIdx.x;
-    for (int i = Wn * Wn + (int)threadIdx.x; i < 2 * Wn * Wn; i += kLanes) win[i] = 0.0f;
     const int gl = lane % G;
     constexpr int NG = kLanes / G;
+    for (int i = Wn * Wn + lane; i < 2 * Wn * Wn; i += kLanes) win[i] = 0.0f;
+    load_homographies(a, Hs);
     const long pid = (long)blockIdx.x;
     const int x = (int)(pid % a.W), y = (int)(pid / a.W);
     const float inv_wn = 1.0f / (float)Wn;
@@ -666,15 +714,13 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
     }
     __syncthreads();
     for (int v0 = 0; v0 < a.V; v0 += NG)
-        eval_pass<G, WN>(a, sb, win, lane, min(NG, a.V - v0), tview + v0, tdep + v0, tout + v0, pc);
+        eval_pass<G, WN, 1>(a, Hs, sb, win, 0, lane, min(NG, a.V - v0), tview + v0, tdep + v0,
+                            tout + v0, nullptr, pc);
     __syncthreads();
     float* cst = a.cost + (size_t)y * a.cost_ld + (size_t)x * a.V;
     if (lane < a.V) cst[lane] = tout[lane];
-    if (lane == 0 && a.counters) {
-        const int nx = pc.npx;
-        atomicAdd(&a.counters[2], (unsigned long long)a.V);
-        atomicAdd(&a.counters[3], (unsigned long long)a.V * (unsigned long long)nx);
-    }
+    /* the evaluation counts of this kernel are a closed form of the image
+     * size (counted on the host): no per-pixel atomics on shared counters */
 }
 
 /* ----------------------------------------------------------- messages */
@@ -733,37 +779,43 @@ __device__ __forceinline__ double msg_forward(double m, double e, const MsgConst
 
 /* ---------------------------------------------------------------- sweep */
 /*
- * topToDown / downToTop (src/patchMatch.cu:818-1209).  One wavefront per
- * sweep column.  Lanes 0..V-1 carry the per-view HMM state; for each row the
- * sampled views are deduplicated and the (view, depth) hypotheses are
- * evaluated four (G=16) at a time.  The NCC of a view at the winning depth
- * is reused when the view was sampled (the reference recomputes the same
- * value).
+ * topToDown / downToTop (src/patchMatch.cu:818-1209).  One workgroup of NW
+ * wavefronts per sweep column.  Every wave runs the column's sequential
+ * logic (HMM messages, posterior, draws, argmin) on identical data, so no
+ * barrier is needed for it; lanes 0..V-1 carry the per-view state.  For
+ * each row the sampled views are deduplicated and the (view, depth)
+ * hypotheses are spread over all NW*64/G lane groups of the workgroup.  The
+ * NCC of a view at the winning depth is reused when the view was sampled
+ * (the reference recomputes the same value).  Wave 0 alone writes the
+ * column's outputs.
  */
-template <int G, int PF, int WN>
-__global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
-    extern __shared__ float smem[];
+template <int G, int PF, int WN, int NW>
+__global__ __launch_bounds__(64 * NW) void pmx_sweep_kernel(PmxSweepArgs a) {
+    extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
-    /* two window buffers, each Wn rows followed by Wn zero rows */
-    const int WW = 2 * Wn * Wn;
-    float* winbuf = smem;                 /* 2 x WW */
-    float* tres = smem + 2 * WW;          /* task results, 2*64 */
-    for (int i = (int)threadIdx.x; i < 2 * WW; i += kLanes) winbuf[i] = 0.0f;
-    int* tview = (int*)(tres + 128);      /* task views, 128 */
-    float* tdep = (float*)(tview + 128);  /* task depths, 128 */
-    int* vj = (int*)(tdep + 128);         /* sampled views, S */
+    const int WW = 2 * Wn * Wn;                          /* window + zero rows */
+    double* Hs = smem_d;                                 /* V x 18 */
+    float4* part = (float4*)(smem_d + a.V * 18);         /* 2 x 64 x NW partials */
+    float* winbuf = (float*)(part + 2 * 64 * NW);        /* 2 x WW */
+    float* tres = winbuf + 2 * WW;                       /* task results, 128 */
+    int* tview = (int*)(tres + 128);                     /* task views, 128 */
+    float* tdep = (float*)(tview + 128);                 /* task depths, 128 */
+    int* vj = (int*)(tdep + 128);                        /* sampled views, S */
 
     constexpr int NG = kLanes / G;
-    const int lane = threadIdx.x;
+    constexpr int NGT = NG * NW;
+    const int wave = (int)threadIdx.x / kLanes, lane = (int)threadIdx.x % kLanes;
+    const bool lead = wave == 0;
     const int gl = lane % G;
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= a.Ws) return;
-    const int V = a.V, Hs = a.Hs;
+    for (int i = (int)threadIdx.x; i < 2 * WW; i += 64 * NW) winbuf[i] = 0.0f;
+    load_homographies(a, Hs);
+    const int V = a.V, Hs_rows = a.Hs;
     const bool is_view = lane < V;
     const int vl = is_view ? lane : 0;
     const float inv_wn = 1.0f / (float)Wn;
     const SrcBuf sb = src_buf(a);
-
     const MsgConst mc = msg_const(a.alpha2);
 
     float* dcol = a.depth + (size_t)c * a.depth_ld;
@@ -778,23 +830,23 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
     /* ---- backward messages (src/patchMatch.cu:858-869, :1050-1061) ---- */
     double m = 0.5;
     {
-        int rb = a.down ? Hs - 1 : 0;
-        const int sb = a.down ? -1 : 1;
+        int rb = a.down ? Hs_rows - 1 : 0;
+        const int sbs = a.down ? -1 : 1;
         float cn = ccol[(size_t)rb * V + vl];
-        for (int k = 0; k < Hs; k++, rb += sb) {
+        for (int k = 0; k < Hs_rows; k++, rb += sbs) {
             const float cst = cn;
-            if (k + 1 < Hs) cn = ccol[(size_t)(rb + sb) * V + vl];
-            if (is_view) scol[(size_t)rb * V + lane] = m;
+            if (k + 1 < Hs_rows) cn = ccol[(size_t)(rb + sbs) * V + vl];
+            if (lead && is_view) scol[(size_t)rb * V + lane] = m;
             const double e = sentinel_emission(cst, is_view, lane, rng);
             m = msg_backward(m, e, mc);
         }
     }
-    /* the backward messages are re-read below by this wavefront */
+    /* the backward messages are re-read below by every wave */
     __threadfence_block();
     __syncthreads();
     /* ---- forward message at the first row ---- */
     const int step = a.down ? 1 : -1;
-    const int r0 = a.down ? 0 : Hs - 1;
+    const int r0 = a.down ? 0 : Hs_rows - 1;
     {
         const float cst = ccol[(size_t)r0 * V + vl];
         const double e = sentinel_emission(cst, is_view, lane, rng);
@@ -831,20 +883,34 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
         pf_stat = a.stats[(size_t)y * a.stats_pitch + x];
         load_window<PF>(a, x, y, qx_col, qy_col, Wn, inv_wn, lane, pf_win);
     };
-    if (Hs > 1) prefetch(r);
+    if (Hs_rows > 1) prefetch(r);
     int buf = 0;
     float D = 0.0f;
-    if (Hs > 1) D = store_window<PF>(winbuf, pf_win, pf_stat.x, Wn, lane, need_d);
+    if (Hs_rows > 1) D = store_window<PF>(winbuf, pf_win, pf_stat.x, Wn, lane, need_d);
 
     unsigned long long n_evals = 0, n_samples = 0;
+    int pb = 0; /* pass counter: parity selects the partial-sum buffer */
 
-    for (int k = 1; k < Hs; k++, r += step) {
+    /* one phase of hypotheses tview/tdep[0..nt) -> tout[0..nt) */
+    auto run_phase = [&](int nt, const float* win, float* tout, const PixelCtx& pc) {
+        for (int t0 = 0; t0 < nt; t0 += NGT) {
+            const int ntp = min(NGT, nt - t0);
+            float4* pp = part + (pb & 1) * 64 * NW;
+            const int kk = eval_pass<G, WN, NW>(a, Hs, sb, win, wave, lane, ntp, tview + t0,
+                                                tdep + t0, tout + t0, pp, pc);
+            __syncthreads();
+            if (kk > NG) combine_pass<NW>(lane, ntp, kk / NG, pp, tout + t0, pc);
+            pb++;
+        }
+    };
+
+    for (int k = 1; k < Hs_rows; k++, r += step) {
         float* win = winbuf + buf * WW;
         __syncthreads();
         const float cst = pf_cost, d_cur = pf_depth;
         const double beta = pf_beta;
         const float2 st = pf_stat;
-        if (k + 1 < Hs) prefetch(r + step);
+        if (k + 1 < Hs_rows) prefetch(r + step);
 
         int x, y;
         orig_xy(r, x, y);
@@ -886,7 +952,7 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
             const int id = hit ? (int)__builtin_ctzll(hit) : V - 1;
             sel |= 1ull << id;
             if (lane == 0) vj[j] = id;
-            if (a.ids && lane == 0)
+            if (lead && a.ids && lane == 0)
                 a.ids[((size_t)j * a.H + y) * a.ids_pitch + x] = (uint8_t)id;
         }
         const int U = __popcll(sel);
@@ -898,14 +964,11 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
             tdep[2 * srank] = d_former;
             tdep[2 * srank + 1] = d_rand;
         }
-        __syncthreads();
+        if (NW == 1) __syncthreads();
 
         /* phase B: former and random depth on the sampled views */
         const int nt = 2 * U;
-        for (int t0 = 0; t0 < nt; t0 += NG)
-            eval_pass<G, WN>(a, sb, win, lane, min(NG, nt - t0), tview + t0, tdep + t0, tres + t0,
-                             pc);
-        __syncthreads();
+        run_phase(nt, win, tres, pc);
         float nf = 0.0f, nr = 0.0f;
         if (in_sel) {
             nf = tres[2 * srank];
@@ -927,7 +990,7 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
         const float cidx = idx == 0 ? c0 : c1s;
         idx = cidx < c2 ? idx : 2;
         const float best = idx == 0 ? d_former : (idx == 1 ? d_cur : d_rand);
-        if (lane == 0) dcol[r] = best;
+        if (lead && lane == 0) dcol[r] = best;
         n_evals += (unsigned long long)nt;
 
         float cnew = cst;
@@ -942,14 +1005,11 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
                 tview[rrank] = lane;
                 tdep[rrank] = best;
             }
-            __syncthreads();
-            for (int t0 = 0; t0 < nrest; t0 += NG)
-                eval_pass<G, WN>(a, sb, win, lane, min(NG, nrest - t0), tview + t0, tdep + t0,
-                                 tres + 64 + t0, pc);
-            __syncthreads();
+            if (NW == 1) __syncthreads();
+            run_phase(nrest, win, tres + 64, pc);
             if (in_sel) cnew = (idx == 0) ? nf : nr;
             if (in_rest) cnew = tres[64 + rrank];
-            if (is_view) ccol[(size_t)r * V + lane] = cnew;
+            if (lead && is_view) ccol[(size_t)r * V + lane] = cnew;
             n_evals += (unsigned long long)nrest;
         }
         n_samples += (unsigned long long)npx * (unsigned long long)(nt + (idx != 1 ? V - U : 0));
@@ -960,12 +1020,12 @@ __global__ __launch_bounds__(64) void pmx_sweep_kernel(PmxSweepArgs a) {
         d_former = best;
 
         /* next window into the other buffer */
-        if (k + 1 < Hs) {
+        if (k + 1 < Hs_rows) {
             buf ^= 1;
             D = store_window<PF>(winbuf + buf * WW, pf_win, pf_stat.x, Wn, lane, need_d);
         }
     }
-    if (lane == 0) {
+    if (lead && lane == 0) {
         xw_store(rp, rng);
         if (a.counters) {
             atomicAdd(&a.counters[0], n_evals);
@@ -1073,17 +1133,18 @@ namespace {
 template <int G, int PF, int WN>
 hipError_t launch_cost_all_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
-    const size_t sh = sizeof(float) * (2 * Wn * Wn + 3 * 64);
+    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float) * (2 * Wn * Wn + 3 * 64);
     hipLaunchKernelGGL((pmx_cost_all_kernel<G, PF, WN>), dim3((unsigned)((long)a.W * a.H)), dim3(64),
                        sh, s, a);
     return hipGetLastError();
 }
 
-template <int G, int PF, int WN>
+template <int G, int PF, int WN, int NW>
 hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
-    const size_t sh = sizeof(float) * (4 * Wn * Wn + 128 + 128 + 128 + 64);
-    hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN>), dim3(a.Ws), dim3(64), sh, s, a);
+    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * 2 * 64 * NW +
+                      sizeof(float) * (4 * Wn * Wn + 128 + 128 + 128 + 64);
+    hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW>), dim3(a.Ws), dim3(64 * NW), sh, s, a);
     return hipGetLastError();
 }
 } // namespace
@@ -1100,11 +1161,18 @@ hipError_t pmx_launch_cost_all(hipStream_t s, const PmxSweepArgs& a) {
     return launch_cost_all_t<64, 44, 0>(s, a);
 }
 
-hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a) {
+hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a, int waves) {
     const int Wn = 2 * a.hw + 1;
-    if (Wn == 15) return launch_sweep_t<16, 4, 15>(s, a);
-    if (Wn == 11) return launch_sweep_t<16, 2, 11>(s, a);
-    if (Wn <= 16) return launch_sweep_t<16, 4, 0>(s, a);
-    if (Wn <= 32) return launch_sweep_t<32, 16, 0>(s, a);
-    return launch_sweep_t<64, 44, 0>(s, a);
+#define PMX_SWEEP_NW(G, PF, WN)                                      \
+    switch (waves) {                                                 \
+    case 1: return launch_sweep_t<G, PF, WN, 1>(s, a);               \
+    case 4: return launch_sweep_t<G, PF, WN, 4>(s, a);               \
+    default: return launch_sweep_t<G, PF, WN, 2>(s, a);              \
+    }
+    if (Wn == 15) { PMX_SWEEP_NW(16, 4, 15) }
+    if (Wn == 11) { PMX_SWEEP_NW(16, 2, 11) }
+    if (Wn <= 16) { PMX_SWEEP_NW(16, 4, 0) }
+    if (Wn <= 32) { PMX_SWEEP_NW(32, 16, 0) }
+    PMX_SWEEP_NW(64, 44, 0)
+#undef PMX_SWEEP_NW
 }
