@@ -15,7 +15,9 @@ Also printed (same JSON line):
   roofline      the sweep kernel's achieved FP32 rate: 21 algorithmic FLOPs
                 per window sample x samples counted on the device / the
                 average sweep launch time from HIP events on the engine's
-                stream, against MI355X's 157.3 TFLOP/s FP32 peak.
+                stream, against MI355X's 157.3 TFLOP/s FP32 peak; traffic =
+                HBM bytes per launch from the committed rocprofv3 counter
+                summary of the same workload (profiles/).
   cpu_baseline  the FP64 oracle (OpenMP) on a crop of the same workload.
   accuracy      the GPU engine on the same crop against the oracle: share of
                 final depths within 1% relative.
@@ -47,6 +49,21 @@ def log(*a):
 def parse_sample(s):
     w, h = s.lower().split("x")
     return int(w), int(h)
+
+
+def profiled_traffic(workload):
+    """HBM bytes per sweep launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    scaled by a copy kernel) and recorded in profiles/; None if not profiled."""
+    path = os.path.join(ROOT, "profiles", f"sweep_kernel_{workload}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if "traffic_bytes_per_launch:" in line:
+                    tok = line.split("traffic_bytes_per_launch:")[1].strip().split(")")[0]
+                    return float(tok), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
 
 
 def run_gpu_steps(eng, steps, warmup):
@@ -129,7 +146,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2_realistic")
-    ap.add_argument("--cpu-sample", default="384x288",
+    ap.add_argument("--cpu-sample", default="768x576",
                     help="reference crop for the CPU oracle baseline and accuracy gate")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads (default: min(16, cpu count))")
@@ -163,6 +180,7 @@ def main():
     avg_ms = st["ms_sweep"] / launches
     flops_per_launch = FLOPS_PER_SAMPLE * st["ncc_samples"] / launches
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    traffic, traffic_src = profiled_traffic(args.workload)
     line = {
         "metric": METRIC,
         "value": value,
@@ -194,7 +212,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "avg_launch_ms": avg_ms,
             "samples_per_launch": st["ncc_samples"] / launches,
             "flops_per_sample": FLOPS_PER_SAMPLE,

@@ -690,8 +690,10 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
     constexpr int NG = kLanes / G;
     for (int i = Wn * Wn + lane; i < 2 * Wn * Wn; i += kLanes) win[i] = 0.0f;
     load_homographies(a, Hs);
-    const long pid = (long)blockIdx.x;
-    const int x = (int)(pid % a.W), y = (int)(pid / a.W);
+    /* contiguous pixel runs per XCD: neighbouring pixels read neighbouring
+     * source footprints, which then stay in that XCD's L2 */
+    const int pid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int x = pid % a.W, y = pid / a.W;
     const float inv_wn = 1.0f / (float)Wn;
     const SrcBuf sb = src_buf(a);
     PixelCtx pc;
