@@ -74,8 +74,8 @@ def build_drop_in():
     srcs = sorted(os.path.join(CPP, f) for f in os.listdir(CPP) if f.endswith(".cpp") and f != "pmx_cli.cpp")
     if not srcs:
         return
-    inc = ["-I", INCLUDE, "-I", CPP]
-    link = [f"-L{HERE}", "-lpmx", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"]
+    inc = ["-I", INCLUDE, "-I", CPP, "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__"]
+    link = [f"-L{HERE}", "-lpmx", "-lz", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"]
     _atomic(LIB_DROPIN, lambda o: [CXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-shared", *inc,
                                    "-o", o, *srcs, *link])
     cli = os.path.join(CPP, "pmx_cli.cpp")
