@@ -205,7 +205,8 @@ int pmx_engine_export(pmx_engine* eng, float** depth, int* depth_pitch,
 int pmx_engine_set_timing(pmx_engine* eng, int enabled);
 /* Launch shape of the sweep kernel: wavefronts per sweep column (1, 2 or
  * 4).  No reference counterpart (its sweeps use one 32-thread block per 32
- * columns, src/patchMatch.cu:334); chosen by the autotuner. */
+ * columns, src/patchMatch.cu:334).  Results do not depend on it beyond
+ * floating-point summation order. */
 int pmx_engine_set_sweep_waves(pmx_engine* eng, int waves);
 int pmx_engine_get_sweep_waves(pmx_engine* eng, int* waves);
 int pmx_engine_get_stats(pmx_engine* eng, pmx_stats* out);
