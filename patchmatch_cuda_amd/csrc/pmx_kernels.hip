@@ -31,6 +31,61 @@ constexpr float kFix = 0.999f;          /* src/patchMatch.cu:9 */
 constexpr float kChange = 1.0f - 0.999f; /* src/patchMatch.cu:10 */
 constexpr float kUniformX = 0.83f;      /* src/patchMatch.cu:673 */
 
+/* ------------------------------------------------------ cross-lane ops */
+/* DPP controls within a 16-lane row: quad permutations, row mirrors and row
+ * shifts.  They run on the VALU (no LDS round trip like ds_bpermute). */
+constexpr int kDppQuadXor1 = 0xB1;      /* quad_perm [1,0,3,2] */
+constexpr int kDppQuadXor2 = 0x4E;      /* quad_perm [2,3,0,1] */
+constexpr int kDppRowMirror = 0x140;
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowShr = 0x110;       /* + n: row_shr:n */
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+
+/* Lanes whose source falls outside the row read 0. */
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+/* Sum over each 16-lane row, the same value in every lane of the row
+ * (each step adds a lane pair symmetrically, so all lanes agree bitwise). */
+__device__ __forceinline__ float row_sum16(float v) {
+    v += dpp_f<kDppQuadXor1>(v);
+    v += dpp_f<kDppQuadXor2>(v);
+    v += dpp_f<kDppRowHalfMirror>(v);
+    v += dpp_f<kDppRowMirror>(v);
+    return v;
+}
+
+/* Inclusive prefix sum over each 16-lane row. */
+__device__ __forceinline__ double row_scan16(double v) {
+    v += dpp_d<kDppRowShr + 1>(v);
+    v += dpp_d<kDppRowShr + 2>(v);
+    v += dpp_d<kDppRowShr + 4>(v);
+    v += dpp_d<kDppRowShr + 8>(v);
+    return v;
+}
+
+/* Value of lane `src` (wave-uniform) in every lane. */
+__device__ __forceinline__ float lane_f(float v, int src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+}
+
+__device__ __forceinline__ double lane_d(double v, int src) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 /* ------------------------------------------------------------------ RNG */
 /* Marsaglia xorwow (J. Stat. Softw. 8(14), 2003); seeding by SplitMix64
  * (Steele, Lea, Flood, OOPSLA 2014) of (seed << 32 | id). */
@@ -352,6 +407,7 @@ struct PixelCtx {
     bool lane_in;        /* this lane's window column is inside */
     float mu, c1, D;
     int npx;             /* in-bounds window pixels n */
+    double inv_n;        /* 1 / n */
     double cx, cy;       /* window centre x + 0.5, y + 0.5 */
     double scale;        /* n . K^-1 (x + 0.5, y + 0.5, 1) */
 };
@@ -506,12 +562,17 @@ __device__ __forceinline__ float ncc_finish(float A, float B, float C, float K,
     /* sum I'^2 == 0 <=> every in-bounds sample is 0; the centre sample K is
      * one of them */
     if (K == 0.0f && B == 0.0f) return kSentinel;
-    const double n = (double)pc.npx;
-    double c2 = (double)B - (double)A * (double)A / n;
+    double c2 = (double)B - (double)A * (double)A * pc.inv_n;
     c2 = c2 < 0.00001 ? 0.0 : c2;
-    const double den = sqrt((double)pc.c1 * c2);
-    if (den == 0.0) return 1.0f;
-    return (float)(1.0 - ((double)C + (double)K * (double)pc.D) / den);
+    /* 1/sqrt(c1 c2): hardware estimate + two Newton steps (both factors are
+     * 0 or >= 1e-5, so p is 0 or a normal number) */
+    const double p = (double)pc.c1 * c2;
+    if (p == 0.0) return 1.0f;
+    double r = __builtin_amdgcn_rsq(p);
+    const double hp = 0.5 * p;
+    r = r * fma(-hp * r, r, 1.5);
+    r = r * fma(-hp * r, r, 1.5);
+    return (float)(1.0 - ((double)C + (double)K * (double)pc.D) * r);
 }
 
 /*
@@ -563,11 +624,17 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
             }
         }
     }
+    if constexpr (G == 16) {
+        A = row_sum16(A);
+        B = row_sum16(B);
+        C = row_sum16(C);
+    } else {
 #pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1) {
-        A += __shfl_xor(A, m, G);
-        B += __shfl_xor(B, m, G);
-        C += __shfl_xor(C, m, G);
+        for (int m = G / 2; m >= 1; m >>= 1) {
+            A += __shfl_xor(A, m, G);
+            B += __shfl_xor(B, m, G);
+            C += __shfl_xor(C, m, G);
+        }
     }
     const int kw = k < NG ? k : NG; /* groups of the hypothesis inside this wave */
     for (int m = G; m < G * kw; m <<= 1) {
@@ -663,6 +730,7 @@ __device__ __forceinline__ void pixel_ctx(const PmxSweepArgs& a, int x, int y, i
     const int nx = min(x + hw, a.W - 1) - max(x - hw, 0) + 1;
     const int ny = pc.dy_hi - pc.dy_lo + 1;
     pc.npx = nx * ny;
+    pc.inv_n = 1.0 / (double)pc.npx;
     pc.cx = (double)x + 0.5;
     pc.cy = (double)y + 0.5;
     pc.scale = a.oX * (a.iK0 * pc.cx + a.iK1) + a.oZ;
@@ -767,7 +835,7 @@ __device__ __forceinline__ double msg_backward(double m, double e, const MsgCons
     const double em = exp(-e * e * mc.inv2a2);
     const double z0 = m * em * fix + (1.0 - m) * mc.up * chg;
     const double z1 = m * em * chg + (1.0 - m) * mc.up * fix;
-    return z0 / (z0 + z1);
+    return z0 / (z0 + z1); /* a true divide: z0 + z1 may be denormal */
 }
 
 /* computeMessageForward (src/patchMatch.cu:692-715) */
@@ -792,7 +860,12 @@ __device__ __forceinline__ double msg_forward(double m, double e, const MsgConst
  * column's outputs.
  */
 template <int G, int PF, int WN, int NW>
-__global__ __launch_bounds__(64 * NW) void pmx_sweep_kernel(PmxSweepArgs a) {
+/* A 2048-column sweep puts 8 workgroups on each of the 256 CUs: with NW
+ * waves each that is 2*NW waves per SIMD, all of which must be resident at
+ * once (a column that waits for a free slot doubles the sweep time), so
+ * the register budget is capped at 512 / (2*NW) per lane. */
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2 * NW > 8 ? 8 : 2 * NW)))
+void pmx_sweep_kernel(PmxSweepArgs a) {
     extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
     const int WW = 2 * Wn * Wn;                          /* window + zero rows */
@@ -932,17 +1005,23 @@ __global__ __launch_bounds__(64 * NW) void pmx_sweep_kernel(PmxSweepArgs a) {
         if (is_view) {
             const double z0 = mf * beta;
             const double z1 = (1.0 - mf) * (1.0 - beta);
+            /* a true divide: with messages near 0 or 1 both terms can be
+             * denormal, where a reciprocal overflows */
             q = z0 / (z0 + z1);
         }
-        /* inclusive prefix sum over lanes 0..V-1 (in view order, as the
-         * sequential sum of src/patchMatch.cu:917-919) */
+        /* inclusive prefix sum over lanes 0..V-1, the normalised CDF of
+         * src/patchMatch.cu:917-923 */
         double cdf = q;
+        if (V <= 16) {
+            cdf = row_scan16(cdf);
+        } else {
 #pragma unroll
-        for (int o = 1; o < kLanes; o <<= 1) {
-            const double t = __shfl_up(cdf, o, kLanes);
-            if (lane >= o) cdf += t;
+            for (int o = 1; o < kLanes; o <<= 1) {
+                const double t = __shfl_up(cdf, o, kLanes);
+                if (lane >= o) cdf += t;
+            }
         }
-        const double total = __shfl(cdf, V - 1, kLanes);
+        const double total = lane_d(cdf, V - 1);
         cdf = (lane == V - 1) ? 1.0 : cdf / total;
 
         /* draws: random depth, then S views (src/patchMatch.cu:937-961) */
@@ -978,10 +1057,10 @@ __global__ __launch_bounds__(64 * NW) void pmx_sweep_kernel(PmxSweepArgs a) {
         }
         float c0 = 0.0f, c1s = 0.0f, c2 = 0.0f;
         for (int j = 0; j < a.S; j++) {
-            const int id = vj[j];
-            c0 += __shfl(nf, id, kLanes);
-            c1s += __shfl(cst, id, kLanes);
-            c2 += __shfl(nr, id, kLanes);
+            const int id = __builtin_amdgcn_readfirstlane(vj[j]);
+            c0 += lane_f(nf, id);
+            c1s += lane_f(cst, id);
+            c2 += lane_f(nr, id);
         }
         const float inv_s = 1.0f / (float)a.S;
         c0 *= inv_s;
