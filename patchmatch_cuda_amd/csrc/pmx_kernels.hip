@@ -26,6 +26,7 @@
 namespace {
 
 constexpr int kLanes = 64;
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
 constexpr float kSentinel = 2.0f;       /* src/patchMatch.cu:594-595 */
 constexpr float kFix = 0.999f;          /* src/patchMatch.cu:9 */
 constexpr float kChange = 1.0f - 0.999f; /* src/patchMatch.cu:10 */
@@ -450,20 +451,35 @@ struct Hyp {
 /* Bilinear tap with zero border (layered texture with border addressing and
  * linear filtering, src/patchMatch.cu:192-195): position relative to the
  * centre texel, clamped into the 2-pixel zero margin of the block. */
-__device__ __forceinline__ float tap(const SrcBuf& sb, const Hyp& q, float sx, float sy) {
+/* A tap in two halves, so that many taps' loads can be in flight before the
+ * first blend: the two 8-byte loads (texel pairs of source rows floor(sy)
+ * and floor(sy)+1) ... */
+__device__ __forceinline__ void tap_fetch(const SrcBuf& sb, const Hyp& q, float sx, float sy,
+                                          float2& t0, float2& t1) {
     sx = __builtin_amdgcn_fmed3f(sx, q.lox, q.hix);
     sy = __builtin_amdgcn_fmed3f(sy, q.loy, q.hiy);
     const int ix = (int)floorf(sx), iy = (int)floorf(sy);
-    const float al = __builtin_amdgcn_fractf(sx), be = __builtin_amdgcn_fractf(sy);
     /* |iy| < 2^17 and the pitch < 2^17: a 24-bit multiply is exact */
     const uint32_t off = (uint32_t)__mul24(iy, (int)sb.pitch_bytes) + ((uint32_t)ix << 2) + q.base;
-    const float2 t0 =
-        __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, 0, 0));
-    const float2 t1 = __builtin_bit_cast(
-        float2, __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, sb.pitch_bytes, 0));
+    t0 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, 0, 0));
+    t1 = __builtin_bit_cast(float2,
+                            __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, sb.pitch_bytes, 0));
+}
+
+/* ... and their bilinear blend with the fractional weights of (sx, sy). */
+__device__ __forceinline__ float tap_blend(const Hyp& q, float sx, float sy, float2 t0, float2 t1) {
+    sx = __builtin_amdgcn_fmed3f(sx, q.lox, q.hix);
+    sy = __builtin_amdgcn_fmed3f(sy, q.loy, q.hiy);
+    const float al = __builtin_amdgcn_fractf(sx), be = __builtin_amdgcn_fractf(sy);
     const float top = fmaf(al, t0.y - t0.x, t0.x);
     const float bot = fmaf(al, t1.y - t1.x, t1.x);
     return fmaf(be, bot - top, top);
+}
+
+__device__ __forceinline__ float tap(const SrcBuf& sb, const Hyp& q, float sx, float sy) {
+    float2 t0, t1;
+    tap_fetch(sb, q, sx, sy, t0, t1);
+    return tap_blend(q, sx, sy, t0, t1);
 }
 
 /* 1/x in FP64: hardware reciprocal refined by two Newton steps (full
@@ -519,7 +535,7 @@ __device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const Src
  * taps of the column can be in flight at once: rows i >= nr still fetch a
  * (clamped, in-bounds) tap and read a zero padding row of the LDS window,
  * and their e is selected to 0. */
-template <int R>
+template <int R, int NB = R>
 __device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q,
                                            const float* __restrict__ win, int Wn, float dxf,
                                            float dy0, const float* __restrict__ wp, int nr,
@@ -527,20 +543,44 @@ __device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q,
     const float nxa = fmaf(q.uy, dy0, q.ux * dxf);
     const float nya = fmaf(q.vy, dy0, q.vx * dxf);
     const float dza = fmaf(q.cy, dy0, fmaf(q.cx, dxf, 1.0f));
-    auto sample = [&](int i) -> float {
+    auto pos = [&](int i) -> float2 {
         const float fi = (float)i;
         const float rz = __builtin_amdgcn_rcpf(fmaf(q.cy, fi, dza));
-        return tap(sb, q, fmaf(fmaf(q.uy, fi, nxa), rz, q.fxc),
-                   fmaf(fmaf(q.vy, fi, nya), rz, q.fyc));
+        return make_float2(fmaf(fmaf(q.uy, fi, nxa), rz, q.fxc), fmaf(fmaf(q.vy, fi, nya), rz, q.fyc));
+    };
+    auto sample = [&](int i) -> float {
+        const float2 p = pos(i);
+        return tap(sb, q, p.x, p.y);
     };
     if constexpr (R > 0) {
+        /* the taps' loads in batches of NB rows, then the blends: the
+         * scheduling barrier keeps the compiler from sinking each load down
+         * to its use, which would make the column R dependent round trips
+         * to memory */
 #pragma unroll
-        for (int i = 0; i < R; i++) {
-            const float val = sample(i);
-            const float e = (i < nr) ? val - q.K : 0.0f;
-            A += e;
-            B = fmaf(e, e, B);
-            C = fmaf(e, wp[i * Wn], C);
+        for (int b0 = 0; b0 < R; b0 += NB) {
+            constexpr int kB = NB;
+            float2 t0[kB], t1[kB];
+#pragma unroll
+            for (int j = 0; j < kB; j++) {
+                if (b0 + j < R) {
+                    const float2 p = pos(b0 + j);
+                    tap_fetch(sb, q, p.x, p.y, t0[j], t1[j]);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < kB; j++) {
+                const int i = b0 + j;
+                if (i < R) {
+                    const float2 p = pos(i);
+                    const float val = tap_blend(q, p.x, p.y, t0[j], t1[j]);
+                    const float e = (i < nr) ? val - q.K : 0.0f;
+                    A += e;
+                    B = fmaf(e, e, B);
+                    C = fmaf(e, wp[i * Wn], C);
+                }
+            }
         }
     } else {
 #pragma unroll 4
@@ -593,6 +633,10 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
                                          const PixelCtx& pc) {
     constexpr int NG = kLanes / G;
     constexpr int NGT = NG * NW;
+    /* tap rows in flight per lane: a whole column with one wave per
+     * workgroup (192 VGPRs, 2 waves per SIMD); 8 rows with more waves,
+     * which must fit 512 / (2 NW) VGPRs */
+    constexpr int kNB = NW == 1 ? 16 : 8;
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
     int k = 1;
     while (2 * k * nt <= NGT) k *= 2;
@@ -613,8 +657,8 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
             const float* wp = win + r0 * Wn + gl;
             if constexpr (WN > 0) {
                 switch (k) {
-                case 1: accum_rows<WN>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
-                case 2: accum_rows<(WN + 1) / 2>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
+                case 1: accum_rows<WN, cmin(WN, kNB)>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
+                case 2: accum_rows<(WN + 1) / 2, cmin((WN + 1) / 2, kNB)>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
                 case 4: accum_rows<(WN + 3) / 4>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
                 case 8: accum_rows<(WN + 7) / 8>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
                 default: accum_rows<(WN + 15) / 16>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
