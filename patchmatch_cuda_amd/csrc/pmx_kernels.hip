@@ -445,7 +445,6 @@ struct Hyp {
     float fxc, fyc;              /* fractional part of the centre */
     float lox, hix, loy, hiy;    /* relative positions that keep a tap inside the padded block */
     uint32_t base;               /* byte offset of texel floor(c') */
-    float K;                     /* source value at the centre */
 };
 
 /* Bilinear tap with zero border (layered texture with border addressing and
@@ -524,7 +523,6 @@ __device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const Src
     q.cy = (float)(h[7] * iz);
     q.base = (uint32_t)view * sb.layer_bytes + (uint32_t)(iyc + PMX_SRC_PAD) * sb.pitch_bytes +
              (uint32_t)(ixc + PMX_SRC_PAD) * 4u;
-    q.K = tap(sb, q, q.fxc, q.fyc);
     return q;
 }
 
@@ -536,10 +534,9 @@ __device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const Src
  * (clamped, in-bounds) tap and read a zero padding row of the LDS window,
  * and their e is selected to 0. */
 template <int R, int NB = R>
-__device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q,
-                                           const float* __restrict__ win, int Wn, float dxf,
+__device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q, int Wn, float dxf,
                                            float dy0, const float* __restrict__ wp, int nr,
-                                           float& A, float& B, float& C) {
+                                           bool valid, float& K, float& A, float& B, float& C) {
     const float nxa = fmaf(q.uy, dy0, q.ux * dxf);
     const float nya = fmaf(q.vy, dy0, q.vx * dxf);
     const float dza = fmaf(q.cy, dy0, fmaf(q.cx, dxf, 1.0f));
@@ -557,10 +554,13 @@ __device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q,
          * scheduling barrier keeps the compiler from sinking each load down
          * to its use, which would make the column R dependent round trips
          * to memory */
+        /* the centre sample K goes out with the first batch */
+        float2 k0, k1;
 #pragma unroll
         for (int b0 = 0; b0 < R; b0 += NB) {
             constexpr int kB = NB;
             float2 t0[kB], t1[kB];
+            if (b0 == 0) tap_fetch(sb, q, q.fxc, q.fyc, k0, k1);
 #pragma unroll
             for (int j = 0; j < kB; j++) {
                 if (b0 + j < R) {
@@ -569,13 +569,14 @@ __device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q,
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
+            if (b0 == 0) K = tap_blend(q, q.fxc, q.fyc, k0, k1);
 #pragma unroll
             for (int j = 0; j < kB; j++) {
                 const int i = b0 + j;
                 if (i < R) {
                     const float2 p = pos(i);
                     const float val = tap_blend(q, p.x, p.y, t0[j], t1[j]);
-                    const float e = (i < nr) ? val - q.K : 0.0f;
+                    const float e = (valid && i < nr) ? val - K : 0.0f;
                     A += e;
                     B = fmaf(e, e, B);
                     C = fmaf(e, wp[i * Wn], C);
@@ -583,9 +584,11 @@ __device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q,
             }
         }
     } else {
+        K = tap(sb, q, q.fxc, q.fyc);
+        if (!valid) return;
 #pragma unroll 4
         for (int i = 0; i < nr; i++) {
-            const float e = sample(i) - q.K;
+            const float e = sample(i) - K;
             A += e;
             B = fmaf(e, e, B);
             C = fmaf(e, wp[i * Wn], C);
@@ -646,26 +649,26 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
     float A = 0.0f, B = 0.0f, C = 0.0f, K = 0.0f;
     if (active) {
         const Hyp q = make_hyp(Hm, sb, tview[t], tdep[t], pc);
-        K = q.K;
         const int R = (Wn + k - 1) / k;
         const int r0 = max(sub * R, pc.dy_lo);
         const int r1 = min(min((sub + 1) * R, Wn), pc.dy_hi + 1);
         const int nr = r1 - r0;
-        if (pc.lane_in && nr > 0) {
-            const float dxf = (float)(gl - a.hw);
-            const float dy0 = (float)(r0 - a.hw);
-            const float* wp = win + r0 * Wn + gl;
-            if constexpr (WN > 0) {
-                switch (k) {
-                case 1: accum_rows<WN, cmin(WN, kNB)>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
-                case 2: accum_rows<(WN + 1) / 2, cmin((WN + 1) / 2, kNB)>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
-                case 4: accum_rows<(WN + 3) / 4>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
-                case 8: accum_rows<(WN + 7) / 8>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
-                default: accum_rows<(WN + 15) / 16>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C); break;
-                }
-            } else {
-                accum_rows<0>(sb, q, win, Wn, dxf, dy0, wp, nr, A, B, C);
+        /* every lane of the group runs the taps (so each holds K); lanes
+         * whose window column or rows fall outside the image add nothing */
+        const bool valid = pc.lane_in && nr > 0;
+        const float dxf = (float)(gl - a.hw);
+        const float dy0 = (float)(r0 - a.hw);
+        const float* wp = win + r0 * Wn + gl;
+        if constexpr (WN > 0) {
+            switch (k) {
+            case 1: accum_rows<WN, cmin(WN, kNB)>(sb, q, Wn, dxf, dy0, wp, nr, valid, K, A, B, C); break;
+            case 2: accum_rows<(WN + 1) / 2, cmin((WN + 1) / 2, kNB)>(sb, q, Wn, dxf, dy0, wp, nr, valid, K, A, B, C); break;
+            case 4: accum_rows<(WN + 3) / 4>(sb, q, Wn, dxf, dy0, wp, nr, valid, K, A, B, C); break;
+            case 8: accum_rows<(WN + 7) / 8>(sb, q, Wn, dxf, dy0, wp, nr, valid, K, A, B, C); break;
+            default: accum_rows<(WN + 15) / 16>(sb, q, Wn, dxf, dy0, wp, nr, valid, K, A, B, C); break;
             }
+        } else {
+            accum_rows<0>(sb, q, Wn, dxf, dy0, wp, nr, valid, K, A, B, C);
         }
     }
     if constexpr (G == 16) {
