@@ -450,35 +450,46 @@ struct Hyp {
 /* Bilinear tap with zero border (layered texture with border addressing and
  * linear filtering, src/patchMatch.cu:192-195): position relative to the
  * centre texel, clamped into the 2-pixel zero margin of the block. */
+/* floor(x) as an integer in one instruction (v_cvt_flr_i32_f32); |x| is
+ * far below 2^31 for every clamped tap position. */
+__device__ __forceinline__ int floor_i(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+/* Fractional weights of a tap, kept from its fetch to its blend. */
+struct TapW {
+    float al, be;
+};
+
 /* A tap in two halves, so that many taps' loads can be in flight before the
  * first blend: the two 8-byte loads (texel pairs of source rows floor(sy)
  * and floor(sy)+1) ... */
-__device__ __forceinline__ void tap_fetch(const SrcBuf& sb, const Hyp& q, float sx, float sy,
+__device__ __forceinline__ TapW tap_fetch(const SrcBuf& sb, const Hyp& q, float sx, float sy,
                                           float2& t0, float2& t1) {
     sx = __builtin_amdgcn_fmed3f(sx, q.lox, q.hix);
     sy = __builtin_amdgcn_fmed3f(sy, q.loy, q.hiy);
-    const int ix = (int)floorf(sx), iy = (int)floorf(sy);
+    const int ix = floor_i(sx), iy = floor_i(sy);
     /* |iy| < 2^17 and the pitch < 2^17: a 24-bit multiply is exact */
-    const uint32_t off = (uint32_t)__mul24(iy, (int)sb.pitch_bytes) + ((uint32_t)ix << 2) + q.base;
+    const uint32_t off = (uint32_t)(__mul24(iy, (int)sb.pitch_bytes) + (int)q.base) + ((uint32_t)ix << 2);
     t0 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, 0, 0));
     t1 = __builtin_bit_cast(float2,
                             __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, sb.pitch_bytes, 0));
+    return TapW{__builtin_amdgcn_fractf(sx), __builtin_amdgcn_fractf(sy)};
 }
 
-/* ... and their bilinear blend with the fractional weights of (sx, sy). */
-__device__ __forceinline__ float tap_blend(const Hyp& q, float sx, float sy, float2 t0, float2 t1) {
-    sx = __builtin_amdgcn_fmed3f(sx, q.lox, q.hix);
-    sy = __builtin_amdgcn_fmed3f(sy, q.loy, q.hiy);
-    const float al = __builtin_amdgcn_fractf(sx), be = __builtin_amdgcn_fractf(sy);
-    const float top = fmaf(al, t0.y - t0.x, t0.x);
-    const float bot = fmaf(al, t1.y - t1.x, t1.x);
-    return fmaf(be, bot - top, top);
+/* ... and their bilinear blend. */
+__device__ __forceinline__ float tap_blend(const TapW& w, float2 t0, float2 t1) {
+    const float top = fmaf(w.al, t0.y - t0.x, t0.x);
+    const float bot = fmaf(w.al, t1.y - t1.x, t1.x);
+    return fmaf(w.be, bot - top, top);
 }
 
 __device__ __forceinline__ float tap(const SrcBuf& sb, const Hyp& q, float sx, float sy) {
     float2 t0, t1;
-    tap_fetch(sb, q, sx, sy, t0, t1);
-    return tap_blend(q, sx, sy, t0, t1);
+    const TapW w = tap_fetch(sb, q, sx, sy, t0, t1);
+    return tap_blend(w, t0, t1);
 }
 
 /* 1/x in FP64: hardware reciprocal refined by two Newton steps (full
@@ -556,26 +567,27 @@ __device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q, int W
          * to memory */
         /* the centre sample K goes out with the first batch */
         float2 k0, k1;
+        TapW kw;
 #pragma unroll
         for (int b0 = 0; b0 < R; b0 += NB) {
             constexpr int kB = NB;
             float2 t0[kB], t1[kB];
-            if (b0 == 0) tap_fetch(sb, q, q.fxc, q.fyc, k0, k1);
+            TapW tw[kB];
+            if (b0 == 0) kw = tap_fetch(sb, q, q.fxc, q.fyc, k0, k1);
 #pragma unroll
             for (int j = 0; j < kB; j++) {
                 if (b0 + j < R) {
                     const float2 p = pos(b0 + j);
-                    tap_fetch(sb, q, p.x, p.y, t0[j], t1[j]);
+                    tw[j] = tap_fetch(sb, q, p.x, p.y, t0[j], t1[j]);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
-            if (b0 == 0) K = tap_blend(q, q.fxc, q.fyc, k0, k1);
+            if (b0 == 0) K = tap_blend(kw, k0, k1);
 #pragma unroll
             for (int j = 0; j < kB; j++) {
                 const int i = b0 + j;
                 if (i < R) {
-                    const float2 p = pos(i);
-                    const float val = tap_blend(q, p.x, p.y, t0[j], t1[j]);
+                    const float val = tap_blend(tw[j], t0[j], t1[j]);
                     const float e = (valid && i < nr) ? val - K : 0.0f;
                     A += e;
                     B = fmaf(e, e, B);
