@@ -212,6 +212,85 @@ int pmx_engine_get_sweep_waves(pmx_engine* eng, int* waves);
 int pmx_engine_get_stats(pmx_engine* eng, pmx_stats* out);
 int pmx_engine_reset_stats(pmx_engine* eng);
 
+/* -------------------------------------------------- learned proposal */
+/* The reference has no learned component.  A proposal is one extra depth
+ * hypothesis per pixel per sweep, a fourth candidate beside the former,
+ * current and random depths of src/patchMatch.cu:827-830 / :935-977; the
+ * averaged NCC cost of findMinCost (:613-618) judges it like the others, and
+ * pixels without a proposal run plain PatchMatch. */
+
+/* Plane-sweep cost volume of the engine's (pre-blurred) images at 1/4
+ * resolution, the network's input: x_dev is a device bf16 tensor of
+ * ceil(H/4) x ceil(W/4) x 64 (NHWC); channel k < 32 is the mean 1 - NCC over
+ * source views view0 and view1 of the window at full-resolution pixel
+ * (min(4X+2, W-1), min(4Y+2, H-1)) on the fronto-parallel plane of inverse
+ * depth 1/far + (1/near - 1/far)(k + 0.5)/32; channel 32 is that pixel's
+ * reference intensity; channels 33..63 are 0.  Needs pmx_engine_prepare. */
+int pmx_engine_cost_volume(pmx_engine* eng, int view0, int view1, void* x_dev);
+/* Sets (depth_dev != NULL) or clears the per-pixel proposal: a device W x H
+ * float plane, 0 = no proposal at that pixel.  Setting it also seeds the
+ * current depth with the proposal where there is one; pmx_engine_prepare
+ * does so again after drawing the initial depth. */
+int pmx_engine_set_proposal(pmx_engine* eng, const float* depth_dev, int pitch_bytes);
+/* numOfIterations after construction (the proposal needs fewer). */
+int pmx_engine_set_num_iterations(pmx_engine* eng, int iterations);
+
+typedef struct pmx_net pmx_net;
+/* The proposal network (4-level BF16 encoder-decoder, 3x3 convolutions with
+ * 32/64/128/256 channels, MFMA kernels), sized for up to max_batch inputs of
+ * H0 x W0 (the 1/4-resolution grid).  Its head predicts inverse depth in
+ * [1/far, 1/near] and a confidence; weights are initialised from seed. */
+int pmx_net_create(pmx_ctx* ctx, int max_batch, int H0, int W0, float depth_near,
+                   float depth_far, unsigned long long seed, pmx_net** out);
+int pmx_net_destroy(pmx_net* net);
+/* Parameters are FP32 master values, all weights ([Cout][9][Cin] per layer,
+ * 15 layers) then all biases; the convolutions read their bf16 rounding. */
+int pmx_net_num_params(const pmx_net* net, size_t* n_total, size_t* n_weights);
+int pmx_net_layer_offsets(const pmx_net* net, size_t* w_off, size_t* b_off); /* 15 each */
+int pmx_net_get_params(pmx_net* net, float* host, size_t n);
+int pmx_net_set_params(pmx_net* net, const float* host, size_t n);
+int pmx_net_get_grads(pmx_net* net, float* host, size_t n);
+int pmx_net_zero_grads(pmx_net* net);
+/* Forward pass of N inputs (device bf16, N x H0 x W0 x 64). */
+int pmx_net_forward(pmx_net* net, const void* x_dev, int N, int H0, int W0);
+/* The last forward's head, N x H0 x W0 x 32 floats (channel 0: inverse-depth
+ * logit, 1: confidence logit). */
+int pmx_net_get_head(pmx_net* net, float* host, size_t n);
+/* Gradients of the last forward's loss, added to the gradient buffer: L1 on
+ * inverse depth against tgt_dev (N x H0 x W0 floats, <= 0 = no target) plus
+ * w_conf x binary cross-entropy of the confidence against "within 1%", both
+ * summed over pixels / (N H0 W0).  loss (host, 2 floats, optional) receives
+ * the two mean terms. */
+int pmx_net_backward(pmx_net* net, const void* x_dev, const float* tgt_dev, float w_conf,
+                     float* loss);
+/* Adam (beta1, beta2, eps) on the accumulated gradients, which it clears. */
+int pmx_net_adam(pmx_net* net, float lr, float beta1, float beta2, float eps);
+/* forward + backward + Adam(0.9, 0.999, 1e-8). */
+int pmx_net_train_step(pmx_net* net, const void* x_dev, const float* tgt_dev, int N, int H0,
+                       int W0, float lr, float w_conf, float* loss);
+/* After a forward of batch 1: the head bilinearly upsampled to W x H; depth =
+ * 1 / inverse depth where the confidence reaches tau, else 0 (no proposal).
+ * conf_dev (optional) receives the confidence. */
+int pmx_net_propose(pmx_net* net, int width, int height, float tau, float* depth_dev,
+                    int depth_pitch_bytes, float* conf_dev, int conf_pitch_bytes);
+/* Cost volume -> forward -> proposal -> pmx_engine_set_proposal. */
+int pmx_engine_propose(pmx_engine* eng, pmx_net* net, int view0, int view1, float tau);
+
+/* Single operators of the network, on device bf16 NHWC tensors (channel
+ * counts multiples of 32, pad 1, stride 1 or 2).  Wt: [Cout][9][Cin]. */
+int pmx_cnn_conv_fwd(pmx_ctx* ctx, const void* X, const void* Wt, const float* bias, void* Y,
+                     float* Yf, int N, int H, int W, int Cin, int Cout, int stride, int relu);
+/* dX (+)= conv3x3^T(dZ); Wt2 = pmx_cnn_weight_t(Wt), [Cin][9][Cout]. */
+int pmx_cnn_conv_bwd_data(pmx_ctx* ctx, const void* dZ, const void* Wt2, void* dX, int N, int H,
+                          int W, int Cin, int Cout, int stride, int accumulate);
+/* dW (FP32, [Cout][9][Cin]) += sum over output pixels of dZ x input patch. */
+int pmx_cnn_conv_bwd_weight(pmx_ctx* ctx, const void* X, const void* dZ, float* dW, int N, int H,
+                            int W, int Cin, int Cout, int stride);
+int pmx_cnn_weight_t(pmx_ctx* ctx, const void* Wt, void* Wt2, int Cout, int Cin);
+/* Training target: mean inverse depth of each 4x4 block of a W x H depth map. */
+int pmx_cnn_target(pmx_ctx* ctx, const float* depth_dev, int depth_pitch_bytes, int width,
+                   int height, float* tgt_dev);
+
 #ifdef __cplusplus
 }
 #endif

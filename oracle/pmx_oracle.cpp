@@ -106,6 +106,8 @@ struct pmxo_problem {
     std::vector<double> H;      /* V*18 */
     std::vector<double> sum_I;  /* W*H */
     std::vector<double> sum_II; /* W*H */
+    /* learned proposal, original frame (W*H) and rotated frame (H*W); empty = none */
+    std::vector<double> prop, prop_t;
     mutable std::atomic<uint64_t> ncc_evals{0};
     mutable std::atomic<uint64_t> samples{0};
 };
@@ -584,7 +586,12 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
 
             const double d_cur = depth_s[(size_t)r * Ws + c];
             const double d_rand = rng.uniform() * (p.far_d - p.near_d) + p.near_d;
-            double cost3[3] = {0.0, 0.0, 0.0};
+            /* a learned proposal (depth > 0) is a fourth candidate on the same
+             * sampled views, compared last */
+            const std::vector<double>& pv = rotated ? pb->prop_t : pb->prop;
+            const double d_prop = pv.empty() ? 0.0 : pv[(size_t)r * Ws + c];
+            const int ncand = d_prop > 0.0 ? 4 : 3;
+            double cost3[4] = {0.0, 0.0, 0.0, 0.0};
             double num_tested = 0.0;
             double vmargin = INFINITY;
             bool clamp_flag = false;
@@ -599,15 +606,19 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
                 cost3[1] += costs[id];
                 cost3[2] += ncc_impl(pb, id, r, c, d_rand, rotated, &clamp_flag, &samples);
                 evals += 2;
+                if (ncand == 4) {
+                    cost3[3] += ncc_impl(pb, id, r, c, d_prop, rotated, &clamp_flag, &samples);
+                    evals += 1;
+                }
             }
             const double inv = 1.0 / num_tested;
-            cost3[0] *= inv;
-            cost3[1] *= inv;
-            cost3[2] *= inv;
-            /* findMinCost (src/patchMatch.cu:613-618). */
+            for (int q = 0; q < 4; q++) cost3[q] *= inv;
+            /* findMinCost (src/patchMatch.cu:613-618): strict <, ties to the
+             * later candidate. */
             int idx = cost3[0] < cost3[1] ? 0 : 1;
             idx = cost3[idx] < cost3[2] ? idx : 2;
-            const double cand[3] = {d_former, d_cur, d_rand};
+            if (ncand == 4) idx = cost3[idx] < cost3[3] ? idx : 3;
+            const double cand[4] = {d_former, d_cur, d_rand, d_prop};
             const double best = cand[idx];
             depth_s[(size_t)r * Ws + c] = best;
             d_former = best;
@@ -621,7 +632,7 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
                 double cm = INFINITY;
                 /* exact sentinel ties (every sampled view out of frame for
                  * both candidates) resolve the same way in any precision */
-                for (int q = 0; q < 3; q++)
+                for (int q = 0; q < ncand; q++)
                     if (q != idx && cand[q] != best &&
                         !(cost3[q] == kSentinel && cost3[idx] == kSentinel))
                         cm = std::fmin(cm, std::fabs(cost3[q] - cost3[idx]));
@@ -672,6 +683,18 @@ void pmxo_run(const pmxo_problem* pb, double* depth, double* cost,
         pmxo_sweep(pb, depth, cost, states, 0, 0, s,
                    (i == iterations - 1) ? used_ids : nullptr, nullptr);
     }
+}
+
+void pmxo_problem_set_proposal(pmxo_problem* pb, const double* prop) {
+    const int W = pb->p.W, H = pb->p.H;
+    if (!prop) {
+        pb->prop.clear();
+        pb->prop_t.clear();
+        return;
+    }
+    pb->prop.assign(prop, prop + (size_t)W * H);
+    pb->prop_t.resize((size_t)W * H);
+    pmxo_transpose(pb->prop.data(), W, H, pb->prop_t.data());
 }
 
 void pmxo_counters(const pmxo_problem* pb, uint64_t* ncc_evals, uint64_t* samples) {

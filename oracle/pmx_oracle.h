@@ -76,7 +76,7 @@ typedef struct pmxo_params {
 typedef struct pmxo_trace {
     double*  view_margin;  /* min_j min_i |u_j - cdf_i| over the S draws */
     double*  cost_margin;  /* min |c_best - c_k| over candidates k with another depth */
-    uint8_t* choice;       /* 0 former, 1 current, 2 random; 255 = row not processed */
+    uint8_t* choice;       /* 0 former, 1 current, 2 random, 3 proposal; 255 = row not processed */
     uint8_t* near_clamp;   /* 1 if any variance term came within 0.1% of 1e-5 */
 } pmxo_trace;
 
@@ -111,6 +111,11 @@ int    pmxo_pick_view(const double* cdf, int V, double u);
 /* Smoothing posterior of one view along a column of L costs, with the
  * message recurrences of src/patchMatch.cu:669-715 (no sentinel costs). */
 void   pmxo_hmm_posteriors(const double* costs, int L, double alpha, double* out);
+/* Learned proposal for the sweeps: W*H depths in the original frame, 0 =
+ * none at that pixel; NULL removes it.  With a proposal a sweep row adds it
+ * as a fourth candidate after former, current and random (no reference
+ * counterpart: the learned proposal of the MI355X engine). */
+void   pmxo_problem_set_proposal(pmxo_problem* pb, const double* prop);
 /* Counters of the last calls (NCC evaluations and window samples). */
 void   pmxo_counters(const pmxo_problem* pb, uint64_t* ncc_evals, uint64_t* samples);
 void   pmxo_set_threads(int n);

@@ -77,6 +77,7 @@ def lib():
                                   ctypes.c_int, _vp, _vp]),
             "pmxo_run": (None, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
             "pmxo_counters": (None, [_vp, _vp, _vp]),
+            "pmxo_problem_set_proposal": (None, [_vp, _vp]),
             "pmxo_pick_view": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
             "pmxo_hmm_posteriors": (None, [_vp, ctypes.c_int, ctypes.c_double, _vp]),
             "pmxo_set_threads": (None, [ctypes.c_int]),
@@ -266,6 +267,16 @@ class Problem:
         ids = np.zeros((S, self.H, self.W), np.uint8) if record_ids else None
         lib().pmxo_run(self._h, _p(depth), _p(cost), _p(states), S, iterations, _p(ids))
         return depth, cost, ids
+
+    def set_proposal(self, prop):
+        """Proposal depths (H x W, original frame, 0 = none) or None."""
+        if prop is None:
+            lib().pmxo_problem_set_proposal(self._h, None)
+            self._prop = None
+            return
+        self._prop = np.ascontiguousarray(prop, np.float64)
+        assert self._prop.shape == (self.H, self.W)
+        lib().pmxo_problem_set_proposal(self._h, _p(self._prop))
 
     def counters(self):
         a = ctypes.c_uint64()

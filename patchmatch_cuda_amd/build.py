@@ -58,7 +58,8 @@ def _atomic(out, cmd_builder):
 
 
 def build_pmx(extra=()):
-    srcs = [os.path.join(CSRC, "pmx_kernels.hip"), os.path.join(CSRC, "pmx_capi.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("pmx_kernels.hip", "pmx_capi.hip", "pmx_cnn.hip",
+                                            "pmx_net.hip")]
     _atomic(LIB_PMX, lambda o: [HIPCC, *HIP_FLAGS, *extra, "-shared", "-o", o, *srcs])
 
 

@@ -120,6 +120,30 @@ SIGNATURES = {
     "pmx_engine_get_sweep_waves": [_vp, ctypes.POINTER(_i)],
     "pmx_engine_get_stats": [_vp, ctypes.POINTER(Stats)],
     "pmx_engine_reset_stats": [_vp],
+    # learned proposal
+    "pmx_engine_cost_volume": [_vp, _i, _i, _vp],
+    "pmx_engine_set_proposal": [_vp, _vp, _i],
+    "pmx_engine_set_num_iterations": [_vp, _i],
+    "pmx_engine_propose": [_vp, _vp, _i, _i, _f],
+    "pmx_net_create": [_vp, _i, _i, _i, _f, _f, ctypes.c_ulonglong, ctypes.POINTER(_vp)],
+    "pmx_net_destroy": [_vp],
+    "pmx_net_num_params": [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
+    "pmx_net_layer_offsets": [_vp, _vp, _vp],
+    "pmx_net_get_params": [_vp, _vp, ctypes.c_size_t],
+    "pmx_net_set_params": [_vp, _vp, ctypes.c_size_t],
+    "pmx_net_get_grads": [_vp, _vp, ctypes.c_size_t],
+    "pmx_net_zero_grads": [_vp],
+    "pmx_net_forward": [_vp, _vp, _i, _i, _i],
+    "pmx_net_get_head": [_vp, _vp, ctypes.c_size_t],
+    "pmx_net_backward": [_vp, _vp, _vp, _f, _vp],
+    "pmx_net_adam": [_vp, _f, _f, _f, _f],
+    "pmx_net_train_step": [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp],
+    "pmx_net_propose": [_vp, _i, _i, _f, _vp, _i, _vp, _i],
+    "pmx_cnn_conv_fwd": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i],
+    "pmx_cnn_conv_bwd_data": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i],
+    "pmx_cnn_conv_bwd_weight": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i],
+    "pmx_cnn_weight_t": [_vp, _vp, _vp, _i, _i],
+    "pmx_cnn_target": [_vp, _vp, _i, _i, _i, _vp],
 }
 _RESTYPES = {"pmx_last_error": ctypes.c_char_p, "pmx_ctx_stream": _vp}
 
@@ -186,6 +210,74 @@ class Context:
     @property
     def stream(self):
         return lib().pmx_ctx_stream(self.h)
+
+
+class DeviceArray:
+    """Device memory from the engine's own allocator (pmx_malloc), with
+    synchronous uploads/downloads and stream-ordered device copies on the
+    context's stream.  dtype is the host view: float32, uint16 (bf16 bits),
+    uint8."""
+
+    def __init__(self, ctx: "Context", shape, dtype=np.float32, zero=True):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.size = int(np.prod(self.shape))
+        self.nbytes = self.size * self.dtype.itemsize
+        h = _vp()
+        ctx.check(lib().pmx_malloc(ctx.h, ctypes.byref(h), max(self.nbytes, 1)))
+        self.h = h
+        if zero:
+            ctx.check(lib().pmx_memset(ctx.h, self.h, 0, self.nbytes))
+
+    @property
+    def ptr(self) -> int:
+        return self.h.value
+
+    def at(self, index) -> int:
+        """Device address of element `index` along the first axis."""
+        return self.h.value + int(index) * (self.nbytes // self.shape[0])
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size == self.size, (a.shape, self.shape)
+        self.ctx.check(lib().pmx_memcpy2d(self.ctx.h, self.h, self.nbytes, _vp(a.ctypes.data),
+                                          self.nbytes, self.nbytes, 1, MEMCPY_H2D))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.check(lib().pmx_memcpy2d(self.ctx.h, _vp(out.ctypes.data), self.nbytes, self.h,
+                                          self.nbytes, self.nbytes, 1, MEMCPY_D2H))
+        return out
+
+    def copy_rows(self, dst_index, src, src_index, count=1):
+        """Stream-ordered copy of `count` leading-axis entries from src."""
+        row = self.nbytes // self.shape[0]
+        assert row == src.nbytes // src.shape[0]
+        assert 0 <= dst_index and dst_index + count <= self.shape[0]
+        assert 0 <= src_index and src_index + count <= src.shape[0]
+        n = row * count
+        self.ctx.check(lib().pmx_memcpy2d(self.ctx.h, _vp(self.at(dst_index)), n,
+                                          _vp(src.at(src_index)), n, n, 1, MEMCPY_D2D))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pmx_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def to_bf16(a) -> np.ndarray:
+    """float -> bf16 bits (uint16), round to nearest even."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def from_bf16(b) -> np.ndarray:
+    return (np.ascontiguousarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
 def _p(a):
@@ -313,6 +405,22 @@ class Engine:
         s = Stats()
         self._c(lib().pmx_engine_get_stats(self.h, ctypes.byref(s)))
         return s.as_dict()
+
+    def cost_volume(self, view0, view1, x_ptr):
+        """Plane-sweep cost volume (network input) into device bf16 memory at x_ptr."""
+        self._c(lib().pmx_engine_cost_volume(self.h, view0, view1, _vp(x_ptr)))
+
+    def set_proposal(self, depth_ptr, pitch_bytes):
+        """Per-pixel proposal depth (device float plane, 0 = none); None clears it."""
+        self._c(lib().pmx_engine_set_proposal(self.h, _vp(depth_ptr) if depth_ptr else None,
+                                              pitch_bytes))
+
+    def set_num_iterations(self, n):
+        self._c(lib().pmx_engine_set_num_iterations(self.h, n))
+
+    def propose(self, net, view0=0, view1=1, tau=0.5):
+        """Cost volume -> network -> proposal, installed for the next run."""
+        self._c(lib().pmx_engine_propose(self.h, net.h, view0, view1, tau))
 
     def reset_stats(self):
         self._c(lib().pmx_engine_reset_stats(self.h))

@@ -185,6 +185,12 @@ PmxSweepArgs sweep_args(pmx_engine* e, int rotated) {
     a.ids = nullptr;
     a.ids_pitch = e->ids_pitch;
     a.counters = e->counters;
+    a.prop = nullptr;
+    a.prop_ld = 0;
+    if (e->has_prop) {
+        a.prop = rotated ? e->prop_rm : e->prop_cm;
+        a.prop_ld = rotated ? e->prop_rm_ld : e->prop_cm_ld;
+    }
     a.W = e->W;
     a.H = e->H;
     a.V = e->V;
@@ -280,6 +286,15 @@ int require_prepared(pmx_engine* e) {
 }
 
 } // namespace
+
+int pmx_fail(pmx_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    return fail(ctx, code, "%s", buf);
+}
 
 extern "C" {
 
@@ -509,7 +524,8 @@ static void engine_free(pmx_engine* e) {
     if (!e) return;
     void* bufs[] = {e->src_raw, e->src,     e->ref_raw,  e->ref,     e->sum_I,    e->sum_II,
                     e->stats,   e->Hm,      e->depth_rm, e->depth_cm, e->cost_rm, e->cost_cm,
-                    e->spm,     e->rng,     e->used_ids, e->out_depth, e->out_cost, e->counters};
+                    e->spm,     e->rng,     e->used_ids, e->out_depth, e->out_cost, e->counters,
+                    e->prop_rm, e->prop_cm};
     for (void* b : bufs) dev_free(b);
     for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
     delete e;
@@ -782,6 +798,11 @@ static int prepare_impl(pmx_engine* e, Timer* tm) {
                                     e->p.rng_seed));
     HIPCHK(ctx, pmx_launch_fill_depth(s, e->rng, e->rng_pitch, e->depth_rm, e->depth_rm_ld, W, H,
                                       e->p.depth_near, e->p.depth_far, e->p.init_mode));
+    /* a learned proposal seeds the depth where it has one (the draws above
+     * keep every pixel's generator in step with the plain method) */
+    if (e->has_prop)
+        HIPCHK(ctx, pmx_launch_seed_depth(s, e->prop_rm, e->prop_rm_ld, e->depth_rm, e->depth_rm_ld,
+                                          W, H));
     if (tm) tm->end(id);
     e->layout_cm = 0;
     e->prepared = true;
@@ -813,6 +834,80 @@ int pmx_engine_run(pmx_engine* e) {
     if (rc == PMX_OK) rc = to_layout(e, 0, &tm);
     tm.flush();
     if (rc == PMX_OK) HIPCHK(e->ctx, hipGetLastError());
+    return rc;
+}
+
+int pmx_engine_set_num_iterations(pmx_engine* e, int iterations) {
+    PMXCHK(check_engine(e));
+    if (iterations < 1) return fail(e->ctx, PMX_ERR_ARG, "iterations must be at least 1");
+    e->p.num_iterations = iterations;
+    return PMX_OK;
+}
+
+int pmx_engine_cost_volume(pmx_engine* e, int view0, int view1, void* x_dev) {
+    PMXCHK(check_engine(e));
+    PMXCHK(require_prepared(e));
+    if (!x_dev) return fail(e->ctx, PMX_ERR_ARG, "x_dev is NULL");
+    if (view0 < 0 || view0 >= e->V || view1 < 0 || view1 >= e->V)
+        return fail(e->ctx, PMX_ERR_ARG, "views %d, %d out of range (%d source views)", view0,
+                    view1, e->V);
+    PmxSweepArgs a = sweep_args(e, 0);
+    HIPCHK(e->ctx, pmx_launch_cost_volume(e->ctx->stream, a, view0, view1, (uint16_t*)x_dev,
+                                          (e->W + 3) / 4, (e->H + 3) / 4));
+    return PMX_OK;
+}
+
+int pmx_engine_set_proposal(pmx_engine* e, const float* depth_dev, int pitch_bytes) {
+    PMXCHK(check_engine(e));
+    pmx_ctx* ctx = e->ctx;
+    if (!depth_dev) {
+        e->has_prop = false;
+        return PMX_OK;
+    }
+    if (pitch_bytes < e->W * 4 || pitch_bytes % 4)
+        return fail(ctx, PMX_ERR_ARG, "pitch %d too small or not a multiple of 4", pitch_bytes);
+    hipStream_t s = ctx->stream;
+    if (!e->prop_rm) {
+        e->prop_rm_ld = round_up(e->W, 64);
+        e->prop_cm_ld = round_up(e->H, 64);
+        PMXCHK(dev_alloc(ctx, (void**)&e->prop_rm, sizeof(float) * (size_t)e->prop_rm_ld * e->H));
+        PMXCHK(dev_alloc(ctx, (void**)&e->prop_cm, sizeof(float) * (size_t)e->prop_cm_ld * e->W));
+    }
+    HIPCHK(ctx, hipMemcpy2DAsync(e->prop_rm, sizeof(float) * e->prop_rm_ld, depth_dev, pitch_bytes,
+                                 sizeof(float) * e->W, e->H, hipMemcpyDeviceToDevice, s));
+    HIPCHK(ctx, pmx_launch_transpose_rec(s, e->prop_rm, e->prop_cm, e->W, e->H, 1, e->prop_rm_ld,
+                                         e->prop_cm_ld));
+    e->has_prop = true;
+    if (e->prepared) {
+        if (e->layout_cm)
+            HIPCHK(ctx, pmx_launch_seed_depth(s, e->prop_cm, e->prop_cm_ld, e->depth_cm,
+                                              e->depth_cm_ld, e->H, e->W));
+        else
+            HIPCHK(ctx, pmx_launch_seed_depth(s, e->prop_rm, e->prop_rm_ld, e->depth_rm,
+                                              e->depth_rm_ld, e->W, e->H));
+    }
+    return PMX_OK;
+}
+
+int pmx_engine_propose(pmx_engine* e, pmx_net* net, int view0, int view1, float tau) {
+    PMXCHK(check_engine(e));
+    PMXCHK(require_prepared(e));
+    if (!net) return fail(e->ctx, PMX_ERR_ARG, "net is NULL");
+    pmx_ctx* ctx = e->ctx;
+    const int W0 = (e->W + 3) / 4, H0 = (e->H + 3) / 4;
+    void* x = nullptr;
+    float* d = nullptr;
+    PMXCHK(dev_alloc(ctx, &x, sizeof(uint16_t) * (size_t)W0 * H0 * 64));
+    int rc = dev_alloc(ctx, (void**)&d, sizeof(float) * (size_t)e->W * e->H);
+    if (rc == PMX_OK) rc = pmx_engine_cost_volume(e, view0, view1, x);
+    if (rc == PMX_OK) rc = pmx_net_forward(net, x, 1, H0, W0);
+    if (rc == PMX_OK) rc = pmx_net_propose(net, e->W, e->H, tau, d, e->W * 4, nullptr, 0);
+    if (rc == PMX_OK) rc = pmx_engine_set_proposal(e, d, e->W * 4);
+    hipError_t es = hipStreamSynchronize(ctx->stream);
+    dev_free(x);
+    dev_free(d);
+    if (rc == PMX_OK && es != hipSuccess)
+        rc = fail(ctx, PMX_ERR_HIP, "%s", hipGetErrorString(es));
     return rc;
 }
 

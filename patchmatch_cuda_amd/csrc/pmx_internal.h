@@ -62,6 +62,8 @@ struct PmxSweepArgs {
     uint8_t* ids;          /* S planes of W x H (original frame) or null */
     int ids_pitch;         /* bytes */
     unsigned long long* counters; /* [0] NCC evaluations, [1] window samples */
+    const float* prop;     /* proposal depth, same layout as depth (0: none); or null */
+    int prop_ld;
     int W, H, V, hw, S;
     int Ws, Hs;            /* sweep frame */
     int down, rotated;
@@ -114,6 +116,11 @@ struct pmx_engine {
     float* out_depth = nullptr; int out_depth_pitch = 0;
     float* out_cost = nullptr;  int out_cost_pitch = 0;
 
+    /* learned proposal: depth per pixel (0 = none), in both layouts */
+    float* prop_rm = nullptr;   int prop_rm_ld = 0;
+    float* prop_cm = nullptr;   int prop_cm_ld = 0;
+    bool has_prop = false;
+
     unsigned long long* counters = nullptr;
     unsigned long long cost_all_evals = 0;   /* closed form, counted on the host */
     unsigned long long cost_all_samples = 0;
@@ -147,9 +154,47 @@ hipError_t pmx_launch_convert_u8(hipStream_t s, const uint8_t* in, int in_pitch,
                                  int out_ld, int width, int height);
 hipError_t pmx_launch_cost_all(hipStream_t s, const PmxSweepArgs& a);
 hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a, int waves);
+hipError_t pmx_launch_cost_volume(hipStream_t s, const PmxSweepArgs& a, int view0, int view1,
+                                  uint16_t* out, int W0, int H0);
+hipError_t pmx_launch_seed_depth(hipStream_t s, const float* prop, int prop_ld, float* depth,
+                                 int depth_ld, int width, int height);
 hipError_t pmx_launch_gather_costs(hipStream_t s, const float* cost_rm, int cost_ld, float* out,
                                    int out_ld, int width, int height, int V);
 hipError_t pmx_launch_scatter_costs(hipStream_t s, const float* in, int in_ld, float* cost_rm,
                                     int cost_ld, int width, int height, int V);
+
+/* ---- learned proposal (pmx_cnn.hip); tensors NHWC bf16 stored as uint16 ---- */
+hipError_t pmx_launch_conv_fwd(hipStream_t s, const uint16_t* X, const uint16_t* W,
+                               const float* bias, uint16_t* Y, float* Yf, int N, int H, int Wd,
+                               int Cin, int Cout, int stride, int relu);
+hipError_t pmx_launch_conv_bwd_data(hipStream_t s, const uint16_t* dZ, const uint16_t* W2,
+                                    uint16_t* dX, int N, int H, int Wd, int Cin, int Cout,
+                                    int stride, int accumulate);
+hipError_t pmx_launch_conv_bwd_weight(hipStream_t s, const uint16_t* X, const uint16_t* dZ,
+                                      float* dW, int N, int H, int Wd, int Cin, int Cout,
+                                      int stride);
+hipError_t pmx_launch_bias_grad(hipStream_t s, const uint16_t* dZ, float* db, int P, int C);
+hipError_t pmx_launch_relu_bwd(hipStream_t s, uint16_t* dY, const uint16_t* Y, size_t n);
+hipError_t pmx_launch_weight_t(hipStream_t s, const uint16_t* W, uint16_t* W2, int Cout, int Cin);
+hipError_t pmx_launch_upcat(hipStream_t s, const uint16_t* A, const uint16_t* S, uint16_t* out,
+                            int N, int H, int W, int Ha, int Wa, int Ca, int Cs);
+hipError_t pmx_launch_upcat_bwd(hipStream_t s, const uint16_t* dO, uint16_t* dA, uint16_t* dS,
+                                int N, int H, int W, int Ha, int Wa, int Ca, int Cs);
+hipError_t pmx_launch_head_loss(hipStream_t s, const float* Hf, const float* tgt, uint16_t* dH,
+                                int P, int C, float lo, float hi, float inv_count, float w_conf,
+                                float* sums);
+hipError_t pmx_launch_adam(hipStream_t s, float* w, float* g, float* m, float* v, uint16_t* wb,
+                           size_t n, size_t n_w, float lr, float b1, float b2, float eps, int t);
+hipError_t pmx_launch_f32_to_bf16(hipStream_t s, const float* a, uint16_t* b, size_t n);
+hipError_t pmx_launch_zero_f32(hipStream_t s, float* a, size_t n);
+hipError_t pmx_launch_proposal(hipStream_t s, const float* Hf, int C, int H0, int W0, int W, int H,
+                               float lo, float hi, float tau, float* out, int ld, float* conf,
+                               int ld_conf);
+hipError_t pmx_launch_target(hipStream_t s, const float* depth, int ld, int W, int H, int W0, int H0,
+                             float* tgt);
+
+/* Error reporting shared by the C ABI sources: records the message for
+ * pmx_last_error and returns code. */
+int pmx_fail(pmx_ctx* ctx, int code, const char* fmt, ...);
 
 #endif

@@ -852,6 +852,72 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
      * size (counted on the host): no per-pixel atomics on shared counters */
 }
 
+/* Plane-sweep cost volume for the proposal network: one wavefront per
+ * 1/4-resolution pixel (X, Y), whose window sits at full-resolution pixel
+ * (min(4X+2, W-1), min(4Y+2, H-1)); its 64 hypotheses are 32 fronto-parallel
+ * planes uniform in inverse depth on two source views.  Output: bf16 NHWC,
+ * 64 channels: mean 1 - NCC of the two views per plane (clamped to [0, 2]),
+ * the reference intensity, zeros. */
+template <int G, int PF, int WN>
+__global__ __launch_bounds__(64) void pmx_volume_kernel(PmxSweepArgs a, int view0, int view1,
+                                                       uint16_t* __restrict__ out, int W0, int H0) {
+    extern __shared__ double smem_d[];
+    const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
+    double* Hs = smem_d;                               /* V x 18 */
+    float* win = (float*)(smem_d + a.V * 18);          /* Wn rows + Wn zero rows, x Wn */
+    int* tview = (int*)(win + 2 * Wn * Wn);            /* 64 */
+    float* tdep = (float*)(tview + 64);                /* 64 */
+    float* tout = tdep + 64;                           /* 64 */
+    const int lane = threadIdx.x;
+    const int gl = lane % G;
+    constexpr int NG = kLanes / G;
+    for (int i = Wn * Wn + lane; i < 2 * Wn * Wn; i += kLanes) win[i] = 0.0f;
+    load_homographies(a, Hs);
+    const int pid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int X = pid % W0, Y = pid / W0;
+    const int x = min(4 * X + 2, a.W - 1), y = min(4 * Y + 2, a.H - 1);
+    const float inv_wn = 1.0f / (float)Wn;
+    const SrcBuf sb = src_buf(a);
+    PixelCtx pc;
+    pixel_ctx(a, x, y, gl, pc);
+    const float2 st = a.stats[(size_t)y * a.stats_pitch + x];
+    pc.mu = st.x;
+    pc.c1 = st.y;
+    const int qx = max(0, a.hw - x);
+    float vals[PF];
+    load_window<PF>(a, x, y, qx, 0, Wn, inv_wn, lane, vals);
+    pc.D = store_window<PF>(win, vals, pc.mu, Wn, lane, qx != 0);
+    {
+        const int k = lane & 31;
+        const float ulo = 1.0f / a.far_d, uhi = 1.0f / a.near_d;
+        tview[lane] = lane < 32 ? view0 : view1;
+        tdep[lane] = 1.0f / (ulo + (uhi - ulo) * ((float)k + 0.5f) * (1.0f / 32.0f));
+    }
+    __syncthreads();
+    for (int t0 = 0; t0 < 64; t0 += NG)
+        eval_pass<G, WN, 1>(a, Hs, sb, win, 0, lane, NG, tview + t0, tdep + t0, tout + t0, nullptr,
+                            pc);
+    __syncthreads();
+    float v = 0.0f;
+    if (lane < 32)
+        v = 0.5f * (fminf(fmaxf(tout[lane], 0.0f), 2.0f) + fminf(fmaxf(tout[32 + lane], 0.0f), 2.0f));
+    else if (lane == 32)
+        v = a.ref[(size_t)y * a.ref_pitch + x];
+    /* bf16, round to nearest even */
+    const uint32_t u = __float_as_uint(v);
+    out[((size_t)Y * W0 + X) * 64 + lane] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+/* depth = prop where prop > 0 (both in the same layout) */
+__global__ void seed_depth_kernel(const float* __restrict__ prop, int prop_ld,
+                                  float* __restrict__ depth, int depth_ld, int width, int height) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= width || y >= height) return;
+    const float p = prop[(size_t)y * prop_ld + x];
+    if (p > 0.0f) depth[(size_t)y * depth_ld + x] = p;
+}
+
 /* ----------------------------------------------------------- messages */
 /* The view-selection HMM runs in FP64: the reference evaluates the emission
  * exp(-0.5 c^2 / alpha^2) in double (src/patchMatch.cu:683, :708), and the
@@ -907,6 +973,10 @@ __device__ __forceinline__ double msg_forward(double m, double e, const MsgConst
 }
 
 /* ---------------------------------------------------------------- sweep */
+/* Hypotheses of one row's phase B: up to 3 per sampled view (former,
+ * random, proposal), at most 64 sampled views. */
+constexpr int kTasks = 3 * 64;
+
 /*
  * topToDown / downToTop (src/patchMatch.cu:818-1209).  One workgroup of NW
  * wavefronts per sweep column.  Every wave runs the column's sequential
@@ -931,10 +1001,10 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
     double* Hs = smem_d;                                 /* V x 18 */
     float4* part = (float4*)(smem_d + a.V * 18);         /* 2 x 64 x NW partials */
     float* winbuf = (float*)(part + 2 * 64 * NW);        /* 2 x WW */
-    float* tres = winbuf + 2 * WW;                       /* task results, 128 */
-    int* tview = (int*)(tres + 128);                     /* task views, 128 */
-    float* tdep = (float*)(tview + 128);                 /* task depths, 128 */
-    int* vj = (int*)(tdep + 128);                        /* sampled views, S */
+    float* tres = winbuf + 2 * WW;                       /* task results: kTasks, then 64 */
+    int* tview = (int*)(tres + kTasks + 64);             /* task views, kTasks */
+    float* tdep = (float*)(tview + kTasks);              /* task depths, kTasks */
+    int* vj = (int*)(tdep + kTasks);                     /* sampled views, S */
 
     constexpr int NG = kLanes / G;
     constexpr int NGT = NG * NW;
@@ -1004,7 +1074,7 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
 
     /* prefetch row r0 + step */
     int r = r0 + step;
-    float pf_cost = 0.0f, pf_depth = 0.0f;
+    float pf_cost = 0.0f, pf_depth = 0.0f, pf_prop = 0.0f;
     double pf_beta = 0.0;
     float2 pf_stat = make_float2(0.0f, 0.0f);
     float pf_win[PF];
@@ -1014,6 +1084,7 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
         pf_cost = ccol[(size_t)rr * V + vl];
         pf_beta = scol[(size_t)rr * V + vl];
         pf_depth = dcol[rr];
+        if (a.prop) pf_prop = a.prop[(size_t)c * a.prop_ld + rr];
         pf_stat = a.stats[(size_t)y * a.stats_pitch + x];
         load_window<PF>(a, x, y, qx_col, qy_col, Wn, inv_wn, lane, pf_win);
     };
@@ -1041,7 +1112,7 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
     for (int k = 1; k < Hs_rows; k++, r += step) {
         float* win = winbuf + buf * WW;
         __syncthreads();
-        const float cst = pf_cost, d_cur = pf_depth;
+        const float cst = pf_cost, d_cur = pf_depth, d_prop = pf_prop;
         const double beta = pf_beta;
         const float2 st = pf_stat;
         if (k + 1 < Hs_rows) prefetch(r + step);
@@ -1098,38 +1169,53 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
         const int U = __popcll(sel);
         const bool in_sel = is_view && ((sel >> lane) & 1ull);
         const int srank = __popcll(sel & ((1ull << lane) - 1ull));
+        /* a learned proposal is a fourth candidate on the same views */
+        const bool has_prop = d_prop > 0.0f;
+        const int per = has_prop ? 3 : 2;
         if (in_sel) {
-            tview[2 * srank] = lane;
-            tview[2 * srank + 1] = lane;
-            tdep[2 * srank] = d_former;
-            tdep[2 * srank + 1] = d_rand;
+            tview[per * srank] = lane;
+            tview[per * srank + 1] = lane;
+            tdep[per * srank] = d_former;
+            tdep[per * srank + 1] = d_rand;
+            if (has_prop) {
+                tview[per * srank + 2] = lane;
+                tdep[per * srank + 2] = d_prop;
+            }
         }
         if (NW == 1) __syncthreads();
 
-        /* phase B: former and random depth on the sampled views */
-        const int nt = 2 * U;
+        /* phase B: former and random depth (and the proposal) on the sampled views */
+        const int nt = per * U;
         run_phase(nt, win, tres, pc);
-        float nf = 0.0f, nr = 0.0f;
+        float nf = 0.0f, nr = 0.0f, np = 0.0f;
         if (in_sel) {
-            nf = tres[2 * srank];
-            nr = tres[2 * srank + 1];
+            nf = tres[per * srank];
+            nr = tres[per * srank + 1];
+            if (has_prop) np = tres[per * srank + 2];
         }
-        float c0 = 0.0f, c1s = 0.0f, c2 = 0.0f;
+        float c0 = 0.0f, c1s = 0.0f, c2 = 0.0f, c3 = 0.0f;
         for (int j = 0; j < a.S; j++) {
             const int id = __builtin_amdgcn_readfirstlane(vj[j]);
             c0 += lane_f(nf, id);
             c1s += lane_f(cst, id);
             c2 += lane_f(nr, id);
+            if (has_prop) c3 += lane_f(np, id);
         }
         const float inv_s = 1.0f / (float)a.S;
         c0 *= inv_s;
         c1s *= inv_s;
         c2 *= inv_s;
-        /* findMinCost (src/patchMatch.cu:613-618) */
+        c3 *= inv_s;
+        /* findMinCost (src/patchMatch.cu:613-618): strict <, ties to the
+         * later candidate; the proposal comes last */
         int idx = c0 < c1s ? 0 : 1;
-        const float cidx = idx == 0 ? c0 : c1s;
+        float cidx = idx == 0 ? c0 : c1s;
         idx = cidx < c2 ? idx : 2;
-        const float best = idx == 0 ? d_former : (idx == 1 ? d_cur : d_rand);
+        if (has_prop) {
+            cidx = idx == 0 ? c0 : (idx == 1 ? c1s : c2);
+            idx = cidx < c3 ? idx : 3;
+        }
+        const float best = idx == 0 ? d_former : (idx == 1 ? d_cur : (idx == 2 ? d_rand : d_prop));
         if (lead && lane == 0) dcol[r] = best;
         n_evals += (unsigned long long)nt;
 
@@ -1146,9 +1232,9 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
                 tdep[rrank] = best;
             }
             if (NW == 1) __syncthreads();
-            run_phase(nrest, win, tres + 64, pc);
-            if (in_sel) cnew = (idx == 0) ? nf : nr;
-            if (in_rest) cnew = tres[64 + rrank];
+            run_phase(nrest, win, tres + kTasks, pc);
+            if (in_sel) cnew = (idx == 0) ? nf : (idx == 2 ? nr : np);
+            if (in_rest) cnew = tres[kTasks + rrank];
             if (lead && is_view) ccol[(size_t)r * V + lane] = cnew;
             n_evals += (unsigned long long)nrest;
         }
@@ -1279,11 +1365,21 @@ hipError_t launch_cost_all_t(hipStream_t s, const PmxSweepArgs& a) {
     return hipGetLastError();
 }
 
+template <int G, int PF, int WN>
+hipError_t launch_volume_t(hipStream_t s, const PmxSweepArgs& a, int v0, int v1, uint16_t* out,
+                           int W0, int H0) {
+    const int Wn = 2 * a.hw + 1;
+    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float) * (2 * Wn * Wn + 3 * 64);
+    hipLaunchKernelGGL((pmx_volume_kernel<G, PF, WN>), dim3((unsigned)((long)W0 * H0)), dim3(64),
+                       sh, s, a, v0, v1, out, W0, H0);
+    return hipGetLastError();
+}
+
 template <int G, int PF, int WN, int NW>
 hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
     const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * 2 * 64 * NW +
-                      sizeof(float) * (4 * Wn * Wn + 128 + 128 + 128 + 64);
+                      sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW>), dim3(a.Ws), dim3(64 * NW), sh, s, a);
     return hipGetLastError();
 }
@@ -1299,6 +1395,24 @@ hipError_t pmx_launch_cost_all(hipStream_t s, const PmxSweepArgs& a) {
     if (Wn <= 16) return launch_cost_all_t<16, 4, 0>(s, a);
     if (Wn <= 32) return launch_cost_all_t<32, 16, 0>(s, a);
     return launch_cost_all_t<64, 44, 0>(s, a);
+}
+
+hipError_t pmx_launch_cost_volume(hipStream_t s, const PmxSweepArgs& a, int view0, int view1,
+                                  uint16_t* out, int W0, int H0) {
+    const int Wn = 2 * a.hw + 1;
+    if (Wn == 15) return launch_volume_t<16, 4, 15>(s, a, view0, view1, out, W0, H0);
+    if (Wn == 11) return launch_volume_t<16, 2, 11>(s, a, view0, view1, out, W0, H0);
+    if (Wn <= 16) return launch_volume_t<16, 4, 0>(s, a, view0, view1, out, W0, H0);
+    if (Wn <= 32) return launch_volume_t<32, 16, 0>(s, a, view0, view1, out, W0, H0);
+    return launch_volume_t<64, 44, 0>(s, a, view0, view1, out, W0, H0);
+}
+
+hipError_t pmx_launch_seed_depth(hipStream_t s, const float* prop, int prop_ld, float* depth,
+                                 int depth_ld, int width, int height) {
+    dim3 b(32, 8);
+    hipLaunchKernelGGL(seed_depth_kernel, grid2(width, height, b), b, 0, s, prop, prop_ld, depth,
+                       depth_ld, width, height);
+    return hipGetLastError();
 }
 
 hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a, int waves) {

@@ -1,0 +1,186 @@
+"""GPU parity of the learned-proposal path: the plane-sweep cost volume
+against the FP64 oracle's NCC, the sweep with a proposal candidate against
+the oracle sweep extended the same way (teacher forced, margin-aware, all
+four directions), a proposal plane of zeros leaving the plain method
+bit-identical, and a ground-truth proposal improving the depth."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import cnn_oracle as co
+from patchmatch_cuda_amd import capi, configs
+
+from pmx_testlib import (TINY, compare_sweep, make_engine, oracle_for, rng_column_states,
+                         to_frame, within)
+
+pytestmark = pytest.mark.gpu
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _plane(ctx, a):
+    a = np.asarray(a, np.float32)
+    return capi.DeviceArray(ctx, a.shape, np.float32).upload(a)
+
+
+@pytest.fixture(scope="module")
+def tiny(gpu_ctx):
+    spec, rp = TINY
+    sc, eng = make_engine(gpu_ctx, spec, rp)
+    yield spec, rp, sc, eng
+    eng.close()
+
+
+@pytest.fixture(scope="module")
+def c1(gpu_ctx):
+    spec, rp = configs.C1_SMALL
+    sc, eng = make_engine(gpu_ctx, spec, rp)
+    yield spec, rp, sc, eng
+    eng.close()
+
+
+def _volume_check(eng, rp, sc, pixels, gpu_ctx):
+    W0, H0 = (eng.W + 3) // 4, (eng.H + 3) // 4
+    x = capi.DeviceArray(gpu_ctx, (H0, W0, 64), np.uint16)
+    eng.prepare()
+    eng.cost_volume(0, 1, x.ptr)
+    g = capi.from_bf16(x.download()).astype(np.float64)
+    pb = oracle_for(eng, rp, sc)
+    ref_img, _ = eng.images()
+    ulo, uhi = 1.0 / np.float32(rp.depth_far), 1.0 / np.float32(rp.depth_near)
+    worst = 0.0
+    for X, Y in pixels:
+        xx, yy = min(4 * X + 2, eng.W - 1), min(4 * Y + 2, eng.H - 1)
+        for k in range(32):
+            d = np.float32(1.0) / np.float32(np.float32(ulo) + np.float32(uhi - ulo) *
+                                             ((np.float32(k) + np.float32(0.5)) / np.float32(32)))
+            c = [min(max(pb.ncc(v, yy, xx, float(d), False), 0.0), 2.0) for v in (0, 1)]
+            ref = 0.5 * (c[0] + c[1])
+            err = abs(g[Y, X, k] - ref)
+            worst = max(worst, err - 0.004 * abs(ref))
+            assert err <= 0.004 * abs(ref) + 2e-5, (X, Y, k, g[Y, X, k], ref)
+        assert g[Y, X, 32] == co.bf16(np.float32(ref_img[yy, xx]))
+        assert not g[Y, X, 33:].any()
+    return worst
+
+
+def test_cost_volume_tiny(tiny, gpu_ctx):
+    spec, rp, sc, eng = tiny
+    _say("case: cost volume tiny (every 1/4-resolution pixel)")
+    W0, H0 = (eng.W + 3) // 4, (eng.H + 3) // 4
+    _volume_check(eng, rp, sc, [(X, Y) for Y in range(H0) for X in range(W0)], gpu_ctx)
+
+
+def test_cost_volume_c1_sample(c1, gpu_ctx):
+    spec, rp, sc, eng = c1
+    _say("case: cost volume c1 (sampled pixels)")
+    rng = np.random.default_rng(0)
+    W0, H0 = (eng.W + 3) // 4, (eng.H + 3) // 4
+    pix = [(0, 0), (W0 - 1, H0 - 1), (0, H0 - 1)] + [
+        (int(rng.integers(W0)), int(rng.integers(H0))) for _ in range(60)]
+    _volume_check(eng, rp, sc, pix, gpu_ctx)
+
+
+def _proposal_plane(sc, seed=1, frac=0.7, jitter=0.03):
+    """Ground truth jittered by up to +-jitter, present on a random share frac
+    of the pixels (0 elsewhere)."""
+    rng = np.random.default_rng(seed)
+    gt = sc.gt_depth.astype(np.float64)
+    p = gt * (1.0 + jitter * rng.uniform(-1, 1, gt.shape))
+    p[rng.uniform(size=gt.shape) > frac] = 0.0
+    return p.astype(np.float32)
+
+
+@pytest.mark.parametrize("down,rotated", [(1, 1), (1, 0), (0, 1), (0, 0)])
+def test_sweep_with_proposal_teacher_forced(tiny, gpu_ctx, down, rotated):
+    spec, rp, sc, eng = tiny
+    _say(f"case: sweep with proposal tiny down={down} rotated={rotated}")
+    prop = _proposal_plane(sc, seed=down * 2 + rotated)
+    pt = _plane(gpu_ctx, prop)
+    eng.prepare()
+    eng.set_proposal(pt.ptr, eng.W * 4)
+    eng.cost_all_views(rotated=True)
+    eng.sweep(1, 1, 1)
+    depth, costs, states = eng.depth(), eng.costs(), eng.rng()
+    seeded = prop > 0
+    S = rp.num_samples
+    pb = oracle_for(eng, rp, sc)
+    pb.set_proposal(prop.astype(np.float64))
+    d_s = to_frame(depth.astype(np.float64), rotated)
+    c_s = to_frame(costs.astype(np.float64), rotated)
+    st_o = states.copy()
+    ids_o, tr = pb.sweep(d_s, c_s, st_o, down, rotated, S, record_ids=True, trace=True)
+    eng.sweep(down, rotated, S, record_ids=True)
+    dg = to_frame(eng.depth(), rotated)
+    ids_g = to_frame(eng.used_ids(), rotated)
+    res = compare_sweep(dg, d_s, ids_g, ids_o, tr, down)
+    chose = np.mean(tr["choice"][tr["choice"] != 255] == 3)
+    _say(f"  compared {res['compared']}/{res['total']} mismatches {res['mismatches']} "
+         f"proposal chosen {chose:.3f} first {res['first_bad']}")
+    eng.set_proposal(None, 0)
+    assert seeded.any()
+    assert res["mismatches"] == 0
+    assert res["compared"] >= 0.6 * res["total"]
+    assert chose > 0.05
+    clean = res["clean_cols"]
+    Ws = d_s.shape[1]
+    a = rng_column_states(eng.rng(), rotated, Ws)[clean].view(np.uint32)
+    b = rng_column_states(st_o, rotated, Ws)[clean].view(np.uint32)
+    assert clean.sum() > 0 and np.array_equal(a, b)
+
+
+def test_set_proposal_seeds_depth(tiny, gpu_ctx):
+    spec, rp, sc, eng = tiny
+    _say("case: proposal seeds the initial depth")
+    eng.prepare()
+    d0 = eng.depth()
+    prop = _proposal_plane(sc, seed=5)
+    pt = _plane(gpu_ctx, prop)
+    eng.set_proposal(pt.ptr, eng.W * 4)
+    d1 = eng.depth()
+    eng.prepare()   # a fresh prepare draws again, then seeds again
+    d2 = eng.depth()
+    eng.set_proposal(None, 0)
+    want = np.where(prop > 0, prop, d0)
+    np.testing.assert_array_equal(d1, want)
+    np.testing.assert_array_equal(d2, want)
+
+
+@pytest.mark.parametrize("which", ["tiny", "c1"])
+def test_zero_proposal_is_plain(request, gpu_ctx, which):
+    spec, rp, sc, eng = request.getfixturevalue(which)
+    _say(f"case: zero proposal == plain ({which})")
+    eng.prepare()
+    eng.run()
+    d_a, c_a, i_a, r_a = eng.depth(), eng.costs(), eng.used_ids(), eng.rng()
+    z = _plane(gpu_ctx, np.zeros((eng.H, eng.W), np.float32))
+    eng.set_proposal(z.ptr, eng.W * 4)
+    eng.prepare()
+    eng.run()
+    d_b, c_b, i_b, r_b = eng.depth(), eng.costs(), eng.used_ids(), eng.rng()
+    eng.set_proposal(None, 0)
+    assert np.array_equal(d_a, d_b)
+    assert np.array_equal(c_a, c_b)
+    assert np.array_equal(i_a, i_b)
+    assert np.array_equal(r_a.view(np.uint32), r_b.view(np.uint32))
+
+
+def test_ground_truth_proposal_improves_depth(c1, gpu_ctx):
+    spec, rp, sc, eng = c1
+    _say("case: ground-truth proposal, 1 iteration vs plain, all iterations")
+    eng.prepare()
+    eng.run()
+    plain = within(eng.depth(), sc.gt_depth, 0.01)
+    g = _plane(gpu_ctx, sc.gt_depth.astype(np.float32))
+    eng.set_proposal(g.ptr, eng.W * 4)
+    eng.set_num_iterations(1)
+    eng.prepare()
+    eng.run()
+    prop = within(eng.depth(), sc.gt_depth, 0.01)
+    eng.set_proposal(None, 0)
+    eng.set_num_iterations(rp.num_iterations)
+    _say(f"  within 1% of ground truth: plain {plain:.4f}, proposal (1 iteration) {prop:.4f}")
+    assert prop > plain
