@@ -227,11 +227,22 @@ int pmx_engine_reset_stats(pmx_engine* eng);
  * depth 1/far + (1/near - 1/far)(k + 0.5)/32; channel 32 is that pixel's
  * reference intensity; channels 33..63 are 0.  Needs pmx_engine_prepare. */
 int pmx_engine_cost_volume(pmx_engine* eng, int view0, int view1, void* x_dev);
-/* Sets (depth_dev != NULL) or clears the per-pixel proposal: a device W x H
- * float plane, 0 = no proposal at that pixel.  Setting it also seeds the
- * current depth with the proposal where there is one; pmx_engine_prepare
- * does so again after drawing the initial depth. */
-int pmx_engine_set_proposal(pmx_engine* eng, const float* depth_dev, int pitch_bytes);
+/* Sets (depth_dev != NULL) or clears the per-pixel proposal: device W x H
+ * float planes of depth (0 = no proposal at that pixel) and spread in
+ * inverse depth (NULL = all 0).  In sweep k (counted from the last prepare)
+ * the proposal hypothesis of pixel (x, y) is depth itself when the spread is
+ * 0, else 1 / clamp(1/depth + spread * h, 1/far, 1/near) with h uniform in
+ * [-1, 1) from an integer hash of (y W + x, k): the pixels' generators are
+ * not touched.  Setting a proposal also seeds the current depth with it
+ * where there is one; pmx_engine_prepare does so again after drawing the
+ * initial depth. */
+int pmx_engine_set_proposal(pmx_engine* eng, const float* depth_dev, const float* spread_dev,
+                            int pitch_bytes);
+/* The two source views farthest from the reference camera (widest
+ * baselines): the cost volume's default pair. */
+int pmx_engine_widest_views(pmx_engine* eng, int* view0, int* view1);
+/* Whether a proposal also seeds the depth (default 1). */
+int pmx_engine_set_proposal_seeding(pmx_engine* eng, int seed_depth);
 /* numOfIterations after construction (the proposal needs fewer). */
 int pmx_engine_set_num_iterations(pmx_engine* eng, int iterations);
 
@@ -257,10 +268,10 @@ int pmx_net_forward(pmx_net* net, const void* x_dev, int N, int H0, int W0);
  * logit, 1: confidence logit). */
 int pmx_net_get_head(pmx_net* net, float* host, size_t n);
 /* Gradients of the last forward's loss, added to the gradient buffer: L1 on
- * inverse depth against tgt_dev (N x H0 x W0 floats, <= 0 = no target) plus
- * w_conf x binary cross-entropy of the confidence against "within 1%", both
- * summed over pixels / (N H0 W0).  loss (host, 2 floats, optional) receives
- * the two mean terms. */
+ * inverse depth u against tgt_dev (N x H0 x W0 floats, <= 0 = no target)
+ * plus w_conf x L1 of the predicted error scale b against |u - t| (error
+ * held fixed), both summed over pixels / (N H0 W0).  loss (host, 2 floats,
+ * optional) receives the two mean terms. */
 int pmx_net_backward(pmx_net* net, const void* x_dev, const float* tgt_dev, float w_conf,
                      float* loss);
 /* Adam (beta1, beta2, eps) on the accumulated gradients, which it clears. */
@@ -268,12 +279,15 @@ int pmx_net_adam(pmx_net* net, float lr, float beta1, float beta2, float eps);
 /* forward + backward + Adam(0.9, 0.999, 1e-8). */
 int pmx_net_train_step(pmx_net* net, const void* x_dev, const float* tgt_dev, int N, int H0,
                        int W0, float lr, float w_conf, float* loss);
-/* After a forward of batch 1: the head bilinearly upsampled to W x H; depth =
- * 1 / inverse depth where the confidence reaches tau, else 0 (no proposal).
- * conf_dev (optional) receives the confidence. */
+/* After a forward of batch 1: the head bilinearly upsampled to W x H.  The
+ * confidence is P(|error| < 1% of u) = 1 - exp(-0.01 u / b); depth = 1/u
+ * where it reaches tau, else 0 (no proposal); spread = 2 b.  spread_dev and
+ * conf_dev are optional. */
 int pmx_net_propose(pmx_net* net, int width, int height, float tau, float* depth_dev,
-                    int depth_pitch_bytes, float* conf_dev, int conf_pitch_bytes);
-/* Cost volume -> forward -> proposal -> pmx_engine_set_proposal. */
+                    int depth_pitch_bytes, float* spread_dev, int spread_pitch_bytes,
+                    float* conf_dev, int conf_pitch_bytes);
+/* Cost volume -> forward -> proposal -> pmx_engine_set_proposal.  view0 or
+ * view1 < 0 selects pmx_engine_widest_views. */
 int pmx_engine_propose(pmx_engine* eng, pmx_net* net, int view0, int view1, float tau);
 
 /* Single operators of the network, on device bf16 NHWC tensors (channel

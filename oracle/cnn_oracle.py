@@ -8,7 +8,7 @@ added to the hypothesis set of src/patchMatch.cu:827-830 / :935-977).  Its
 operators are textbook: 3x3 convolution with zero padding 1 and stride 1 or 2
 (NHWC activations, weights [Cout][3][3][Cin]), ReLU, nearest x2 upsampling
 with concatenation, a sigmoid-parameterised inverse-depth head with L1 loss
-and a binary cross-entropy confidence.  bf16() emulates the GPU's rounding of
+and an error scale regressed (L1) onto the detached error.  bf16() emulates the GPU's rounding of
 stored activations and gradients (round to nearest even) so that the two
 passes can be compared step for step; pass round_bf16=False for the exact
 FP64 network used to pin the backward pass by finite differences.
@@ -99,25 +99,35 @@ def sigmoid(z):
     return 1.0 / (1.0 + np.exp(-z))
 
 
-def head_loss(Hf, tgt, lo, hi, w_conf):
-    """L1 on inverse depth u = lo + (hi - lo) sigmoid(z0) against tgt (> 0 only)
-    plus w_conf x BCE of sigmoid(z1) against |u - tgt| < 0.01 tgt, both summed
-    over pixels / P.  Returns (dHf, mean L1, mean BCE)."""
+SCALE_LO, SCALE_HI = 1e-4, 0.5
+
+
+def head_outputs(Hf, lo, hi):
+    """Inverse depth u = lo + (hi - lo) sigmoid(z0) and the scale of its
+    error b = SCALE_LO + (SCALE_HI - SCALE_LO) sigmoid(z1)."""
+    u = lo + (hi - lo) * sigmoid(Hf[..., 0])
+    b = SCALE_LO + (SCALE_HI - SCALE_LO) * sigmoid(Hf[..., 1])
+    return u, b
+
+
+def head_loss(Hf, tgt, lo, hi, w_conf, e_fixed=None):
+    """L1 on u against tgt (> 0 only) plus w_conf x |b - |e|| with the error
+    e = u - tgt held fixed (its gradient reaches b only), both summed over
+    pixels / P.  e_fixed replaces e in the scale term (for finite differences
+    of that surrogate).  Returns (dHf, mean L1, mean scale L1)."""
     P = Hf.shape[0] * Hf.shape[1] * Hf.shape[2]
-    z0, z1 = Hf[..., 0], Hf[..., 1]
-    s0 = sigmoid(z0)
-    u = lo + (hi - lo) * s0
-    c = sigmoid(z1)
+    s0, s1 = sigmoid(Hf[..., 0]), sigmoid(Hf[..., 1])
+    u, b = head_outputs(Hf, lo, hi)
     valid = tgt > 0
     e = u - tgt
-    y = (np.abs(e) < 0.01 * tgt).astype(np.float64)
+    ae = np.abs(e if e_fixed is None else e_fixed)
     dH = np.zeros_like(Hf)
     dH[..., 0] = np.where(valid, np.sign(e) * (hi - lo) * s0 * (1 - s0) / P, 0.0)
-    dH[..., 1] = np.where(valid, w_conf * (c - y) / P, 0.0)
+    dH[..., 1] = np.where(valid, w_conf * np.sign(b - ae) * (SCALE_HI - SCALE_LO) * s1 * (1 - s1)
+                          / P, 0.0)
     l1 = np.where(valid, np.abs(e), 0.0).sum() / P
-    bce = np.where(valid, -(y * np.log(np.maximum(c, 1e-7)) +
-                            (1 - y) * np.log(np.maximum(1 - c, 1e-7))), 0.0).sum() / P
-    return dH, l1, bce
+    sl1 = np.where(valid, np.abs(b - ae), 0.0).sum() / P
+    return dH, l1, sl1
 
 
 def unpack(params, layers=LAYERS):

@@ -106,8 +106,10 @@ struct pmxo_problem {
     std::vector<double> H;      /* V*18 */
     std::vector<double> sum_I;  /* W*H */
     std::vector<double> sum_II; /* W*H */
-    /* learned proposal, original frame (W*H) and rotated frame (H*W); empty = none */
-    std::vector<double> prop, prop_t;
+    /* learned proposal (depth, spread), original frame (W*H) and rotated
+     * frame (H*W); empty = none */
+    std::vector<double> prop, prop_t, spread, spread_t;
+    mutable int sweep_id = 0; /* sweeps since the proposal was set */
     mutable std::atomic<uint64_t> ncc_evals{0};
     mutable std::atomic<uint64_t> samples{0};
 };
@@ -516,6 +518,30 @@ void pmxo_cost_all_views(const pmxo_problem* pb, const double* depth_s,
     pb->samples += tot;
 }
 
+/* lowbias32 integer hash (C. Wellons, public domain constants). */
+static uint32_t mix32(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+/* The proposal hypothesis of pixel (x, y) in sweep k: inverse depth
+ * 1/depth + spread h, h uniform in [-1, 1) from the hash of (y W + x, k),
+ * clamped to [1/far, 1/near], all in FP32 with correctly rounded division
+ * (the engine's arithmetic, reproduced bit for bit). */
+static double proposal_draw(float depth, float spread, int x, int y, const pmxo_params& p,
+                            int sweep_id) {
+    const uint32_t k = mix32((uint32_t)(y * p.W + x) ^
+                             mix32((uint32_t)sweep_id * 0x9E3779B9u + 0x632BE5ABu));
+    const float h = (float)(k >> 8) * (1.0f / 8388608.0f) - 1.0f;
+    const float lo = 1.0f / (float)p.far_d, hi = 1.0f / (float)p.near_d;
+    const float u = std::fmin(std::fmax(std::fmaf(spread, h, 1.0f / depth), lo), hi);
+    return (double)(1.0f / u);
+}
+
 /* topToDown (src/patchMatch.cu:818-1010) and downToTop (:1012-1209). */
 void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
                 pmxo_rng* states, int down, int rotated, int S,
@@ -589,7 +615,11 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
             /* a learned proposal (depth > 0) is a fourth candidate on the same
              * sampled views, compared last */
             const std::vector<double>& pv = rotated ? pb->prop_t : pb->prop;
-            const double d_prop = pv.empty() ? 0.0 : pv[(size_t)r * Ws + c];
+            const std::vector<double>& sv = rotated ? pb->spread_t : pb->spread;
+            double d_prop = pv.empty() ? 0.0 : pv[(size_t)r * Ws + c];
+            if (d_prop > 0.0 && sv[(size_t)r * Ws + c] > 0.0)
+                d_prop = proposal_draw((float)d_prop, (float)sv[(size_t)r * Ws + c],
+                                       rotated ? r : c, rotated ? c : r, p, pb->sweep_id);
             const int ncand = d_prop > 0.0 ? 4 : 3;
             double cost3[4] = {0.0, 0.0, 0.0, 0.0};
             double num_tested = 0.0;
@@ -650,6 +680,7 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
     }
     pb->ncc_evals += tot_evals;
     pb->samples += tot_samples;
+    pb->sweep_id++;
 }
 
 /* PatchMatch::run<W> (src/patchMatch.cu:307-407) including
@@ -685,16 +716,34 @@ void pmxo_run(const pmxo_problem* pb, double* depth, double* cost,
     }
 }
 
-void pmxo_problem_set_proposal(pmxo_problem* pb, const double* prop) {
+void pmxo_problem_set_proposal(pmxo_problem* pb, const double* prop, const double* spread) {
     const int W = pb->p.W, H = pb->p.H;
+    pb->sweep_id = 0;
     if (!prop) {
         pb->prop.clear();
         pb->prop_t.clear();
+        pb->spread.clear();
+        pb->spread_t.clear();
         return;
     }
     pb->prop.assign(prop, prop + (size_t)W * H);
+    if (spread) pb->spread.assign(spread, spread + (size_t)W * H);
+    else pb->spread.assign((size_t)W * H, 0.0);
     pb->prop_t.resize((size_t)W * H);
+    pb->spread_t.resize((size_t)W * H);
     pmxo_transpose(pb->prop.data(), W, H, pb->prop_t.data());
+    pmxo_transpose(pb->spread.data(), W, H, pb->spread_t.data());
+}
+
+void pmxo_problem_set_sweep_id(pmxo_problem* pb, int sweep_id) { pb->sweep_id = sweep_id; }
+
+double pmxo_proposal_draw(float depth, float spread, int x, int y, int W, double near_d,
+                          double far_d, int sweep_id) {
+    pmxo_params p{};
+    p.W = W;
+    p.near_d = near_d;
+    p.far_d = far_d;
+    return spread > 0.0f ? proposal_draw(depth, spread, x, y, p, sweep_id) : (double)depth;
 }
 
 void pmxo_counters(const pmxo_problem* pb, uint64_t* ncc_evals, uint64_t* samples) {

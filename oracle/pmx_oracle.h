@@ -111,11 +111,18 @@ int    pmxo_pick_view(const double* cdf, int V, double u);
 /* Smoothing posterior of one view along a column of L costs, with the
  * message recurrences of src/patchMatch.cu:669-715 (no sentinel costs). */
 void   pmxo_hmm_posteriors(const double* costs, int L, double alpha, double* out);
-/* Learned proposal for the sweeps: W*H depths in the original frame, 0 =
- * none at that pixel; NULL removes it.  With a proposal a sweep row adds it
- * as a fourth candidate after former, current and random (no reference
- * counterpart: the learned proposal of the MI355X engine). */
-void   pmxo_problem_set_proposal(pmxo_problem* pb, const double* prop);
+/* Learned proposal for the sweeps: W*H depths in the original frame (0 =
+ * none at that pixel) and spreads in inverse depth (NULL = 0); NULL depths
+ * remove it.  A sweep row adds the proposal as a fourth candidate after
+ * former, current and random: the depth itself for spread 0, else a draw
+ * keyed by the pixel and the sweep count (no reference counterpart: the
+ * learned proposal of the MI355X engine).  Resets the sweep count to 0;
+ * every pmxo_sweep advances it. */
+void   pmxo_problem_set_proposal(pmxo_problem* pb, const double* prop, const double* spread);
+void   pmxo_problem_set_sweep_id(pmxo_problem* pb, int sweep_id);
+/* The proposal hypothesis of pixel (x, y) of a W-wide image in sweep k. */
+double pmxo_proposal_draw(float depth, float spread, int x, int y, int W, double near_d,
+                          double far_d, int sweep_id);
 /* Counters of the last calls (NCC evaluations and window samples). */
 void   pmxo_counters(const pmxo_problem* pb, uint64_t* ncc_evals, uint64_t* samples);
 void   pmxo_set_threads(int n);

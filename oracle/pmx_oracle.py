@@ -77,7 +77,11 @@ def lib():
                                   ctypes.c_int, _vp, _vp]),
             "pmxo_run": (None, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
             "pmxo_counters": (None, [_vp, _vp, _vp]),
-            "pmxo_problem_set_proposal": (None, [_vp, _vp]),
+            "pmxo_problem_set_proposal": (None, [_vp, _vp, _vp]),
+            "pmxo_problem_set_sweep_id": (None, [_vp, ctypes.c_int]),
+            "pmxo_proposal_draw": (ctypes.c_double, [ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                     ctypes.c_double, ctypes.c_int]),
             "pmxo_pick_view": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
             "pmxo_hmm_posteriors": (None, [_vp, ctypes.c_int, ctypes.c_double, _vp]),
             "pmxo_set_threads": (None, [ctypes.c_int]),
@@ -198,6 +202,10 @@ def hmm_posteriors(costs, alpha):
     return out
 
 
+def proposal_draw(depth, spread, x, y, W, near, far, sweep_id) -> float:
+    return float(lib().pmxo_proposal_draw(depth, spread, x, y, W, near, far, sweep_id))
+
+
 def inv_k4(K: np.ndarray):
     Ki = np.linalg.inv(np.asarray(K, np.float64).reshape(3, 3))
     return [Ki[0, 0], Ki[0, 2], Ki[1, 1], Ki[1, 2]]
@@ -268,15 +276,22 @@ class Problem:
         lib().pmxo_run(self._h, _p(depth), _p(cost), _p(states), S, iterations, _p(ids))
         return depth, cost, ids
 
-    def set_proposal(self, prop):
-        """Proposal depths (H x W, original frame, 0 = none) or None."""
+    def set_proposal(self, prop, spread=None):
+        """Proposal depths (H x W, original frame, 0 = none) and spreads in
+        inverse depth (or None = 0); prop None removes it.  Resets the sweep
+        count that keys the proposal draws."""
         if prop is None:
-            lib().pmxo_problem_set_proposal(self._h, None)
+            lib().pmxo_problem_set_proposal(self._h, None, None)
             self._prop = None
             return
         self._prop = np.ascontiguousarray(prop, np.float64)
         assert self._prop.shape == (self.H, self.W)
-        lib().pmxo_problem_set_proposal(self._h, _p(self._prop))
+        self._spread = None if spread is None else np.ascontiguousarray(spread, np.float64)
+        lib().pmxo_problem_set_proposal(self._h, _p(self._prop),
+                                        _p(self._spread) if self._spread is not None else None)
+
+    def set_sweep_id(self, k):
+        lib().pmxo_problem_set_sweep_id(self._h, int(k))
 
     def counters(self):
         a = ctypes.c_uint64()

@@ -122,8 +122,10 @@ SIGNATURES = {
     "pmx_engine_reset_stats": [_vp],
     # learned proposal
     "pmx_engine_cost_volume": [_vp, _i, _i, _vp],
-    "pmx_engine_set_proposal": [_vp, _vp, _i],
+    "pmx_engine_set_proposal": [_vp, _vp, _vp, _i],
+    "pmx_engine_widest_views": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "pmx_engine_set_num_iterations": [_vp, _i],
+    "pmx_engine_set_proposal_seeding": [_vp, _i],
     "pmx_engine_propose": [_vp, _vp, _i, _i, _f],
     "pmx_net_create": [_vp, _i, _i, _i, _f, _f, ctypes.c_ulonglong, ctypes.POINTER(_vp)],
     "pmx_net_destroy": [_vp],
@@ -138,7 +140,7 @@ SIGNATURES = {
     "pmx_net_backward": [_vp, _vp, _vp, _f, _vp],
     "pmx_net_adam": [_vp, _f, _f, _f, _f],
     "pmx_net_train_step": [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp],
-    "pmx_net_propose": [_vp, _i, _i, _f, _vp, _i, _vp, _i],
+    "pmx_net_propose": [_vp, _i, _i, _f, _vp, _i, _vp, _i, _vp, _i],
     "pmx_cnn_conv_fwd": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i],
     "pmx_cnn_conv_bwd_data": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i],
     "pmx_cnn_conv_bwd_weight": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i],
@@ -410,15 +412,25 @@ class Engine:
         """Plane-sweep cost volume (network input) into device bf16 memory at x_ptr."""
         self._c(lib().pmx_engine_cost_volume(self.h, view0, view1, _vp(x_ptr)))
 
-    def set_proposal(self, depth_ptr, pitch_bytes):
-        """Per-pixel proposal depth (device float plane, 0 = none); None clears it."""
+    def set_proposal(self, depth_ptr, pitch_bytes, spread_ptr=None):
+        """Per-pixel proposal depth (device float plane, 0 = none) and optional
+        spread in inverse depth; depth_ptr None clears it."""
         self._c(lib().pmx_engine_set_proposal(self.h, _vp(depth_ptr) if depth_ptr else None,
+                                              _vp(spread_ptr) if spread_ptr else None,
                                               pitch_bytes))
+
+    def widest_views(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._c(lib().pmx_engine_widest_views(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_proposal_seeding(self, on=True):
+        self._c(lib().pmx_engine_set_proposal_seeding(self.h, int(bool(on))))
 
     def set_num_iterations(self, n):
         self._c(lib().pmx_engine_set_num_iterations(self.h, n))
 
-    def propose(self, net, view0=0, view1=1, tau=0.5):
+    def propose(self, net, view0=-1, view1=-1, tau=0.0):
         """Cost volume -> network -> proposal, installed for the next run."""
         self._c(lib().pmx_engine_propose(self.h, net.h, view0, view1, tau))
 

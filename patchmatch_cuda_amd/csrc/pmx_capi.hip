@@ -186,9 +186,12 @@ PmxSweepArgs sweep_args(pmx_engine* e, int rotated) {
     a.ids_pitch = e->ids_pitch;
     a.counters = e->counters;
     a.prop = nullptr;
+    a.prop_spread = nullptr;
     a.prop_ld = 0;
+    a.sweep_id = e->sweep_count;
     if (e->has_prop) {
         a.prop = rotated ? e->prop_rm : e->prop_cm;
+        a.prop_spread = rotated ? e->spread_rm : e->spread_cm;
         a.prop_ld = rotated ? e->prop_rm_ld : e->prop_cm_ld;
     }
     a.W = e->W;
@@ -271,6 +274,7 @@ int do_sweep(pmx_engine* e, int down, int rotated, int S, bool ids, Timer* tm) {
     HIPCHK(e->ctx, pmx_launch_sweep(e->ctx->stream, a, e->sweep_waves));
     if (tm) tm->end(id);
     e->stats_acc.sweep_launches++;
+    e->sweep_count++;
     return PMX_OK;
 }
 
@@ -525,7 +529,7 @@ static void engine_free(pmx_engine* e) {
     void* bufs[] = {e->src_raw, e->src,     e->ref_raw,  e->ref,     e->sum_I,    e->sum_II,
                     e->stats,   e->Hm,      e->depth_rm, e->depth_cm, e->cost_rm, e->cost_cm,
                     e->spm,     e->rng,     e->used_ids, e->out_depth, e->out_cost, e->counters,
-                    e->prop_rm, e->prop_cm};
+                    e->prop_rm, e->prop_cm, e->spread_rm, e->spread_cm};
     for (void* b : bufs) dev_free(b);
     for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
     delete e;
@@ -800,12 +804,13 @@ static int prepare_impl(pmx_engine* e, Timer* tm) {
                                       e->p.depth_near, e->p.depth_far, e->p.init_mode));
     /* a learned proposal seeds the depth where it has one (the draws above
      * keep every pixel's generator in step with the plain method) */
-    if (e->has_prop)
+    if (e->has_prop && e->prop_seeds)
         HIPCHK(ctx, pmx_launch_seed_depth(s, e->prop_rm, e->prop_rm_ld, e->depth_rm, e->depth_rm_ld,
                                           W, H));
     if (tm) tm->end(id);
     e->layout_cm = 0;
     e->prepared = true;
+    e->sweep_count = 0;
     return PMX_OK;
 }
 
@@ -837,6 +842,36 @@ int pmx_engine_run(pmx_engine* e) {
     return rc;
 }
 
+/* The two source views whose centres are farthest from the reference's
+ * (the widest baselines), for the cost volume. */
+static void widest_views(const pmx_engine* e, int* v0, int* v1) {
+    std::vector<std::pair<double, int>> b;
+    for (int v = 0; v < e->V; v++) {
+        double s = 0.0;
+        for (int k = 0; k < 3; k++) {
+            const double d = e->src_cams[v].C[k] - e->ref_cam.C[k];
+            s += d * d;
+        }
+        b.push_back({-s, v});
+    }
+    std::sort(b.begin(), b.end());
+    *v0 = b[0].second;
+    *v1 = b.size() > 1 ? b[1].second : b[0].second;
+}
+
+int pmx_engine_widest_views(pmx_engine* e, int* view0, int* view1) {
+    PMXCHK(check_engine(e));
+    if (!view0 || !view1) return fail(e->ctx, PMX_ERR_ARG, "NULL argument");
+    widest_views(e, view0, view1);
+    return PMX_OK;
+}
+
+int pmx_engine_set_proposal_seeding(pmx_engine* e, int seed_depth) {
+    PMXCHK(check_engine(e));
+    e->prop_seeds = seed_depth != 0;
+    return PMX_OK;
+}
+
 int pmx_engine_set_num_iterations(pmx_engine* e, int iterations) {
     PMXCHK(check_engine(e));
     if (iterations < 1) return fail(e->ctx, PMX_ERR_ARG, "iterations must be at least 1");
@@ -857,7 +892,8 @@ int pmx_engine_cost_volume(pmx_engine* e, int view0, int view1, void* x_dev) {
     return PMX_OK;
 }
 
-int pmx_engine_set_proposal(pmx_engine* e, const float* depth_dev, int pitch_bytes) {
+int pmx_engine_set_proposal(pmx_engine* e, const float* depth_dev, const float* spread_dev,
+                            int pitch_bytes) {
     PMXCHK(check_engine(e));
     pmx_ctx* ctx = e->ctx;
     if (!depth_dev) {
@@ -872,13 +908,23 @@ int pmx_engine_set_proposal(pmx_engine* e, const float* depth_dev, int pitch_byt
         e->prop_cm_ld = round_up(e->H, 64);
         PMXCHK(dev_alloc(ctx, (void**)&e->prop_rm, sizeof(float) * (size_t)e->prop_rm_ld * e->H));
         PMXCHK(dev_alloc(ctx, (void**)&e->prop_cm, sizeof(float) * (size_t)e->prop_cm_ld * e->W));
+        PMXCHK(dev_alloc(ctx, (void**)&e->spread_rm, sizeof(float) * (size_t)e->prop_rm_ld * e->H));
+        PMXCHK(dev_alloc(ctx, (void**)&e->spread_cm, sizeof(float) * (size_t)e->prop_cm_ld * e->W));
     }
     HIPCHK(ctx, hipMemcpy2DAsync(e->prop_rm, sizeof(float) * e->prop_rm_ld, depth_dev, pitch_bytes,
                                  sizeof(float) * e->W, e->H, hipMemcpyDeviceToDevice, s));
+    if (spread_dev)
+        HIPCHK(ctx, hipMemcpy2DAsync(e->spread_rm, sizeof(float) * e->prop_rm_ld, spread_dev,
+                                     pitch_bytes, sizeof(float) * e->W, e->H,
+                                     hipMemcpyDeviceToDevice, s));
+    else
+        HIPCHK(ctx, hipMemsetAsync(e->spread_rm, 0, sizeof(float) * (size_t)e->prop_rm_ld * e->H, s));
     HIPCHK(ctx, pmx_launch_transpose_rec(s, e->prop_rm, e->prop_cm, e->W, e->H, 1, e->prop_rm_ld,
                                          e->prop_cm_ld));
+    HIPCHK(ctx, pmx_launch_transpose_rec(s, e->spread_rm, e->spread_cm, e->W, e->H, 1,
+                                         e->prop_rm_ld, e->prop_cm_ld));
     e->has_prop = true;
-    if (e->prepared) {
+    if (e->prepared && e->prop_seeds) {
         if (e->layout_cm)
             HIPCHK(ctx, pmx_launch_seed_depth(s, e->prop_cm, e->prop_cm_ld, e->depth_cm,
                                               e->depth_cm_ld, e->H, e->W));
@@ -895,14 +941,17 @@ int pmx_engine_propose(pmx_engine* e, pmx_net* net, int view0, int view1, float 
     if (!net) return fail(e->ctx, PMX_ERR_ARG, "net is NULL");
     pmx_ctx* ctx = e->ctx;
     const int W0 = (e->W + 3) / 4, H0 = (e->H + 3) / 4;
+    if (view0 < 0 || view1 < 0) widest_views(e, &view0, &view1);
     void* x = nullptr;
     float* d = nullptr;
     PMXCHK(dev_alloc(ctx, &x, sizeof(uint16_t) * (size_t)W0 * H0 * 64));
-    int rc = dev_alloc(ctx, (void**)&d, sizeof(float) * (size_t)e->W * e->H);
+    int rc = dev_alloc(ctx, (void**)&d, 2 * sizeof(float) * (size_t)e->W * e->H);
+    float* sp = d + (size_t)e->W * e->H;
     if (rc == PMX_OK) rc = pmx_engine_cost_volume(e, view0, view1, x);
     if (rc == PMX_OK) rc = pmx_net_forward(net, x, 1, H0, W0);
-    if (rc == PMX_OK) rc = pmx_net_propose(net, e->W, e->H, tau, d, e->W * 4, nullptr, 0);
-    if (rc == PMX_OK) rc = pmx_engine_set_proposal(e, d, e->W * 4);
+    if (rc == PMX_OK)
+        rc = pmx_net_propose(net, e->W, e->H, tau, d, e->W * 4, sp, e->W * 4, nullptr, 0);
+    if (rc == PMX_OK) rc = pmx_engine_set_proposal(e, d, sp, e->W * 4);
     hipError_t es = hipStreamSynchronize(ctx->stream);
     dev_free(x);
     dev_free(d);

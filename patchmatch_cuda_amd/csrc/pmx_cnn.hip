@@ -331,9 +331,14 @@ __global__ void upcat_bwd_kernel(const uint16_t* __restrict__ dO, uint16_t* __re
     }
 }
 
-/* Head output (FP32, channel 0: inverse-depth logit, 1: confidence logit)
- * -> L1 loss on inverse depth + BCE on "within 1%" confidence; gradient
- * into dH (bf16, C channels, zero beyond 1).  tgt <= 0 marks no target. */
+/* Head output (FP32, channel 0: inverse-depth logit z0, 1: scale logit z1):
+ * inverse depth u = lo + (hi - lo) sigmoid(z0), scale of its error
+ * b = kScaleLo + (kScaleHi - kScaleLo) sigmoid(z1).  Loss: L1 on u plus
+ * w_conf x L1 of b against the error |u - t| held fixed (b learns the median
+ * error; both gradients are bounded).  Gradient into dH (bf16, C channels,
+ * zero beyond 1).  tgt <= 0 marks no target. */
+constexpr float kScaleLo = 1e-4f, kScaleHi = 0.5f;
+
 __global__ void head_loss_kernel(const float* __restrict__ Hf, const float* __restrict__ tgt,
                                  uint16_t* __restrict__ dH, int P, int C, float lo, float hi,
                                  float inv_count, float w_conf, float* __restrict__ sums) {
@@ -343,16 +348,18 @@ __global__ void head_loss_kernel(const float* __restrict__ Hf, const float* __re
         const float z0 = Hf[(size_t)p * C], z1 = Hf[(size_t)p * C + 1];
         const float s0 = 1.0f / (1.0f + __expf(-z0));
         const float u = lo + (hi - lo) * s0;
-        const float c = 1.0f / (1.0f + __expf(-z1));
+        const float s1 = 1.0f / (1.0f + __expf(-z1));
+        const float b = kScaleLo + (kScaleHi - kScaleLo) * s1;
         const float t = tgt[p];
         float g0 = 0.0f, g1 = 0.0f;
         if (t > 0.0f) {
             const float e = u - t;
             l1 = fabsf(e);
             g0 = (e > 0.0f ? 1.0f : (e < 0.0f ? -1.0f : 0.0f)) * (hi - lo) * s0 * (1.0f - s0) * inv_count;
-            const float y = fabsf(e) < 0.01f * t ? 1.0f : 0.0f;
-            bce = -(y * logf(fmaxf(c, 1e-7f)) + (1.0f - y) * logf(fmaxf(1.0f - c, 1e-7f)));
-            g1 = w_conf * (c - y) * inv_count;
+            const float eb = b - fabsf(e);
+            bce = fabsf(eb);
+            g1 = w_conf * (eb > 0.0f ? 1.0f : (eb < 0.0f ? -1.0f : 0.0f)) * (kScaleHi - kScaleLo) *
+                 s1 * (1.0f - s1) * inv_count;
         }
         for (int ch = 0; ch < C; ch++)
             dH[(size_t)p * C + ch] = f2bf(ch == 0 ? g0 : (ch == 1 ? g1 : 0.0f));
@@ -398,10 +405,16 @@ __global__ void zero_f32_kernel(float* a, size_t n) {
 }
 
 /* Proposal at full resolution: bilinear x4 upsampling (clamp to edge) of the
- * head's inverse depth and confidence; depth = 1/u where the confidence
- * reaches tau, else 0 (no proposal).  out_conf optional. */
+ * head's inverse depth u and error scale b.  Confidence = 1 - exp(-0.01 u / b),
+ * the chance of an error below 1% of u for an exponential error of scale b.  depth = 1/u where the
+ * confidence reaches tau, else 0 (no proposal); spread = kSpread b, the
+ * half-width in inverse depth of the sweeps' draws around u.  out_spread and
+ * out_conf optional. */
+constexpr float kSpread = 2.0f;
+
 __global__ void proposal_kernel(const float* __restrict__ Hf, int C, int H0, int W0, int W, int H,
                                 float lo, float hi, float tau, float* __restrict__ out, int ld,
+                                float* __restrict__ out_spread, int ld_spread,
                                 float* __restrict__ out_conf, int ld_conf) {
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
@@ -419,10 +432,13 @@ __global__ void proposal_kernel(const float* __restrict__ Hf, int C, int H0, int
             const float* h = Hf + ((size_t)ys[j] * W0 + xs[i]) * C;
             const float w = wx[i] * wy[j];
             u += w * (lo + (hi - lo) / (1.0f + __expf(-h[0])));
-            c += w / (1.0f + __expf(-h[1]));
+            c += w * (kScaleLo + (kScaleHi - kScaleLo) / (1.0f + __expf(-h[1])));
         }
-    out[(size_t)y * ld + x] = (c >= tau && u > 0.0f) ? 1.0f / u : 0.0f;
-    if (out_conf) out_conf[(size_t)y * ld_conf + x] = c;
+    const float b = c;
+    const float conf = 1.0f - __expf(-0.01f * u / b);
+    out[(size_t)y * ld + x] = (conf >= tau && u > 0.0f) ? 1.0f / u : 0.0f;
+    if (out_spread) out_spread[(size_t)y * ld_spread + x] = kSpread * b;
+    if (out_conf) out_conf[(size_t)y * ld_conf + x] = conf;
 }
 
 /* Training target: mean inverse depth of each 4x4 block of a depth map
@@ -545,10 +561,11 @@ hipError_t pmx_launch_zero_f32(hipStream_t s, float* a, size_t n) {
 }
 
 hipError_t pmx_launch_proposal(hipStream_t s, const float* Hf, int C, int H0, int W0, int W, int H,
-                               float lo, float hi, float tau, float* out, int ld, float* conf,
-                               int ld_conf) {
+                               float lo, float hi, float tau, float* out, int ld, float* spread,
+                               int ld_spread, float* conf, int ld_conf) {
     dim3 b(32, 8), g(nblk(W, 32), nblk(H, 8));
-    proposal_kernel<<<g, b, 0, s>>>(Hf, C, H0, W0, W, H, lo, hi, tau, out, ld, conf, ld_conf);
+    proposal_kernel<<<g, b, 0, s>>>(Hf, C, H0, W0, W, H, lo, hi, tau, out, ld, spread, ld_spread,
+                                    conf, ld_conf);
     return hipGetLastError();
 }
 

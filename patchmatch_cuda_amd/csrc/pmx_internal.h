@@ -63,7 +63,9 @@ struct PmxSweepArgs {
     int ids_pitch;         /* bytes */
     unsigned long long* counters; /* [0] NCC evaluations, [1] window samples */
     const float* prop;     /* proposal depth, same layout as depth (0: none); or null */
+    const float* prop_spread; /* its spread in inverse depth, same layout (0: exact) */
     int prop_ld;
+    int sweep_id;          /* sweeps since prepare: keys the proposal draws */
     int W, H, V, hw, S;
     int Ws, Hs;            /* sweep frame */
     int down, rotated;
@@ -119,7 +121,11 @@ struct pmx_engine {
     /* learned proposal: depth per pixel (0 = none), in both layouts */
     float* prop_rm = nullptr;   int prop_rm_ld = 0;
     float* prop_cm = nullptr;   int prop_cm_ld = 0;
+    float* spread_rm = nullptr; /* proposal spread, same layouts and lds */
+    float* spread_cm = nullptr;
     bool has_prop = false;
+    bool prop_seeds = true;     /* the proposal seeds the depth */
+    int sweep_count = 0;        /* sweeps since the last prepare */
 
     unsigned long long* counters = nullptr;
     unsigned long long cost_all_evals = 0;   /* closed form, counted on the host */
@@ -188,8 +194,8 @@ hipError_t pmx_launch_adam(hipStream_t s, float* w, float* g, float* m, float* v
 hipError_t pmx_launch_f32_to_bf16(hipStream_t s, const float* a, uint16_t* b, size_t n);
 hipError_t pmx_launch_zero_f32(hipStream_t s, float* a, size_t n);
 hipError_t pmx_launch_proposal(hipStream_t s, const float* Hf, int C, int H0, int W0, int W, int H,
-                               float lo, float hi, float tau, float* out, int ld, float* conf,
-                               int ld_conf);
+                               float lo, float hi, float tau, float* out, int ld, float* spread,
+                               int ld_spread, float* conf, int ld_conf);
 hipError_t pmx_launch_target(hipStream_t s, const float* depth, int ld, int W, int H, int W0, int H0,
                              float* tgt);
 

@@ -977,6 +977,30 @@ __device__ __forceinline__ double msg_forward(double m, double e, const MsgConst
  * random, proposal), at most 64 sampled views. */
 constexpr int kTasks = 3 * 64;
 
+/* lowbias32 integer hash (C. Wellons, public domain constants). */
+__device__ __forceinline__ uint32_t mix32(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+/* The proposal's hypothesis of one pixel in one sweep: inverse depth
+ * 1/depth + spread * h, h uniform in [-1, 1) from a hash of the pixel and
+ * the sweep (the column's generator is not touched), clamped to
+ * [1/far, 1/near].  FP32 with correctly rounded divisions, so the oracle
+ * reproduces it bit for bit. */
+__device__ __forceinline__ float proposal_draw(float depth, float spread, int x, int y,
+                                               const PmxSweepArgs& a) {
+    const uint32_t k = mix32((uint32_t)(y * a.W + x) ^
+                             mix32((uint32_t)a.sweep_id * 0x9E3779B9u + 0x632BE5ABu));
+    const float h = (float)(k >> 8) * (1.0f / 8388608.0f) - 1.0f;
+    const float u = fminf(fmaxf(fmaf(spread, h, 1.0f / depth), 1.0f / a.far_d), 1.0f / a.near_d);
+    return 1.0f / u;
+}
+
 /*
  * topToDown / downToTop (src/patchMatch.cu:818-1209).  One workgroup of NW
  * wavefronts per sweep column.  Every wave runs the column's sequential
@@ -1074,7 +1098,7 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
 
     /* prefetch row r0 + step */
     int r = r0 + step;
-    float pf_cost = 0.0f, pf_depth = 0.0f, pf_prop = 0.0f;
+    float pf_cost = 0.0f, pf_depth = 0.0f, pf_prop = 0.0f, pf_spread = 0.0f;
     double pf_beta = 0.0;
     float2 pf_stat = make_float2(0.0f, 0.0f);
     float pf_win[PF];
@@ -1084,7 +1108,10 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
         pf_cost = ccol[(size_t)rr * V + vl];
         pf_beta = scol[(size_t)rr * V + vl];
         pf_depth = dcol[rr];
-        if (a.prop) pf_prop = a.prop[(size_t)c * a.prop_ld + rr];
+        if (a.prop) {
+            pf_prop = a.prop[(size_t)c * a.prop_ld + rr];
+            pf_spread = a.prop_spread[(size_t)c * a.prop_ld + rr];
+        }
         pf_stat = a.stats[(size_t)y * a.stats_pitch + x];
         load_window<PF>(a, x, y, qx_col, qy_col, Wn, inv_wn, lane, pf_win);
     };
@@ -1112,13 +1139,15 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
     for (int k = 1; k < Hs_rows; k++, r += step) {
         float* win = winbuf + buf * WW;
         __syncthreads();
-        const float cst = pf_cost, d_cur = pf_depth, d_prop = pf_prop;
+        const float cst = pf_cost, d_cur = pf_depth, spread = pf_spread;
         const double beta = pf_beta;
+        float d_prop = pf_prop;
         const float2 st = pf_stat;
         if (k + 1 < Hs_rows) prefetch(r + step);
 
         int x, y;
         orig_xy(r, x, y);
+        if (d_prop > 0.0f && spread > 0.0f) d_prop = proposal_draw(d_prop, spread, x, y, a);
         PixelCtx pc;
         pixel_ctx(a, x, y, gl, pc);
         pc.mu = st.x;

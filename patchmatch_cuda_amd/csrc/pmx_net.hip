@@ -474,18 +474,18 @@ int pmx_net_train_step(pmx_net* net, const void* x, const float* tgt, int N, int
 }
 
 int pmx_net_propose(pmx_net* net, int W, int H, float tau, float* depth, int depth_pitch,
-                    float* conf, int conf_pitch) {
+                    float* spread, int spread_pitch, float* conf, int conf_pitch) {
     if (!net || !depth) return pmx_fail(net ? net->ctx : nullptr, PMX_ERR_ARG, "NULL argument");
     if (net->N != 1)
         return pmx_fail(net->ctx, PMX_ERR_STATE, "propose needs a forward pass of batch 1");
     if ((W + 3) / 4 != net->fW0 || (H + 3) / 4 != net->fH0)
         return pmx_fail(net->ctx, PMX_ERR_ARG, "%dx%d does not match the forward pass (%dx%d)", W,
                         H, net->fW0, net->fH0);
-    if (depth_pitch < W * 4 || (conf && conf_pitch < W * 4))
+    if (depth_pitch < W * 4 || (conf && conf_pitch < W * 4) || (spread && spread_pitch < W * 4))
         return pmx_fail(net->ctx, PMX_ERR_ARG, "pitch smaller than a row");
     NCHK(net, pmx_launch_proposal(net->ctx->stream, net->head, kHeadC, net->fH0, net->fW0, W, H,
-                                  net->inv_lo, net->inv_hi, tau, depth, depth_pitch / 4, conf,
-                                  conf_pitch / 4));
+                                  net->inv_lo, net->inv_hi, tau, depth, depth_pitch / 4, spread,
+                                  spread_pitch / 4, conf, conf_pitch / 4));
     return PMX_OK;
 }
 

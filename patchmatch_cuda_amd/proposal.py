@@ -9,9 +9,13 @@ turns that into one extra depth hypothesis per pixel per sweep; the NCC cost
 judges it with the other candidates (src/patchMatch.cu:613-618) and pixels
 below the confidence threshold run plain PatchMatch.
 
+The head predicts inverse depth u and the scale b of its error; the sweeps
+draw one hypothesis per pixel per sweep from 1/depth +- 2b, and the
+confidence 1 - exp(-0.01 u / b) gates where a proposal is made.
+
 Training data: each scene is rendered (patchmatch_cuda_amd.scene), the plain
 engine runs on it, and the 4x4-block mean inverse depth of its depth map is
-the target (L1), with "within 1%" as the confidence label (cross-entropy).
+the target: L1 on u plus 0.1 x L1 of b against the detached error |u - t|.
 Adam, learning rate 1e-3, batch 8.
 
 Device memory comes from the engine's allocator (capi.DeviceArray); every
@@ -136,11 +140,11 @@ class Net:
                                               _vp(loss.ctypes.data) if want_loss else None))
         return loss
 
-    def propose(self, W, H, tau, depth, conf=None):
-        """depth, conf: DeviceArray H x W float32."""
-        self._c(capi.lib().pmx_net_propose(self.h, W, H, tau, _vp(depth.ptr), W * 4,
-                                           _vp(conf.ptr) if conf is not None else None,
-                                           W * 4 if conf is not None else 0))
+    def propose(self, W, H, tau, depth, spread=None, conf=None):
+        """depth, spread, conf: DeviceArray H x W float32."""
+        opt = lambda a: (_vp(a.ptr), W * 4) if a is not None else (None, 0)  # noqa: E731
+        self._c(capi.lib().pmx_net_propose(self.h, W, H, tau, _vp(depth.ptr), W * 4, *opt(spread),
+                                           *opt(conf)))
 
     # weights file: bf16 weights + FP32 biases, float32 .npy (no pickle)
     def save(self, path):
@@ -197,7 +201,7 @@ def make_dataset(ctx, n, W=640, H=480, V=6, seed0=10_000, threads=16, log=print)
             eng = capi.Engine(ctx, rp, sc.ref, sc.srcs, sc.cams)
             eng.prepare()
             eng.run()
-            eng.cost_volume(0, 1, x_all.at(i))
+            eng.cost_volume(*eng.widest_views(), x_all.at(i))
             depth.upload(eng.depth())
             ctx.check(capi.lib().pmx_cnn_target(ctx.h, _vp(depth.ptr), W * 4, W, H,
                                                 _vp(t_all.at(i))))

@@ -94,18 +94,18 @@ def test_network_gradient_matches_finite_differences():
     x = np.zeros((1, 6, 5, 64))
     x[..., :33] = rng.uniform(0, 1, (1, 6, 5, 33))
     lo, hi = 0.05, 1.0
-    # targets far from any prediction's 1% band: the confidence label is 0
-    # everywhere, so the loss is smooth in the parameters
+    # targets away from every prediction, so the L1 term is smooth
     tgt = np.full((1, 6, 5), 0.3)
     tgt[0, 0, 0] = 0.0  # no target
     act = co.forward(params, x, round_bf16=False)
-    u = lo + (hi - lo) * co.sigmoid(act[14][..., 0])
+    u, _ = co.head_outputs(act[14], lo, hi)
     assert np.all(np.abs(u - 0.3) > 0.02)
+    e0 = u - tgt   # the likelihood term's error is held at its base value
 
     def loss(p):
         a = co.forward(p, x, round_bf16=False)
-        _, l1, bce = co.head_loss(a[14], tgt, lo, hi, 0.5)
-        return l1 + 0.5 * bce
+        _, l1, nll = co.head_loss(a[14], tgt, lo, hi, 0.5, e_fixed=e0)
+        return l1 + 0.5 * nll
 
     g, _, _ = co.backward(params, act, tgt, lo, hi, 0.5, round_bf16=False)
     offs = np.cumsum([0] + [c * 9 * i for i, c, _, _ in co.LAYERS])

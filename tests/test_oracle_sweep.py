@@ -102,3 +102,42 @@ def test_cost_all_views_counts(setup):
     span = lambda n: sum(min(i + hw, n - 1) - max(i - hw, 0) + 1 for i in range(n))
     assert e1 - e0 == W * H * pb.V
     assert s1 - s0 == span(W) * span(H) * pb.V
+
+
+def _mix32(v):
+    v = np.uint64(v) & np.uint64(0xFFFFFFFF)
+    v ^= v >> np.uint64(16)
+    v = (v * np.uint64(0x7feb352d)) & np.uint64(0xFFFFFFFF)
+    v ^= v >> np.uint64(15)
+    v = (v * np.uint64(0x846ca68b)) & np.uint64(0xFFFFFFFF)
+    v ^= v >> np.uint64(16)
+    return v
+
+
+def test_proposal_draw_restated():
+    """The proposal hypothesis: an independent numpy restatement of the hash
+    and the FP32 arithmetic, spread 0 = the depth itself, draws inside
+    [near, far], and h spread evenly over [-1, 1)."""
+    W, near, far = 37, np.float32(1.0), np.float32(20.0)
+    f = np.float32
+    hs = []
+    for k in range(3):
+        for y in range(5):
+            for x in range(W):
+                depth, spread = f(2.0 + 0.1 * x), f(0.05)
+                key = _mix32(np.uint64(y * W + x) ^ _mix32(
+                    (np.uint64(k) * np.uint64(0x9E3779B9) + np.uint64(0x632BE5AB)) & np.uint64(0xFFFFFFFF)))
+                h = f(int(key >> np.uint64(8))) * f(1.0 / 8388608.0) - f(1.0)
+                # fmaf: the product of two floats is exact in double, and the
+                # sum is rounded once more to float
+                u = f(float(spread) * float(h) + float(f(1.0) / depth))
+                u = min(max(u, f(1.0) / far), f(1.0) / near)
+                want = float(f(1.0) / f(u))
+                got = O.proposal_draw(depth, spread, x, y, W, float(near), float(far), k)
+                assert got == want, (k, x, y, got, want)
+                assert float(near) <= got <= float(far)
+                hs.append(float(h))
+                assert O.proposal_draw(depth, 0.0, x, y, W, 1.0, 20.0, k) == float(depth)
+    hs = np.array(hs)
+    assert hs.min() >= -1.0 and hs.max() < 1.0
+    assert abs(hs.mean()) < 0.1 and np.histogram(hs, 4, (-1, 1))[0].min() > len(hs) / 8

@@ -86,29 +86,31 @@ def test_cost_volume_c1_sample(c1, gpu_ctx):
 
 def _proposal_plane(sc, seed=1, frac=0.7, jitter=0.03):
     """Ground truth jittered by up to +-jitter, present on a random share frac
-    of the pixels (0 elsewhere)."""
+    of the pixels (0 elsewhere); half of the pixels get a spread (draws)."""
     rng = np.random.default_rng(seed)
     gt = sc.gt_depth.astype(np.float64)
     p = gt * (1.0 + jitter * rng.uniform(-1, 1, gt.shape))
     p[rng.uniform(size=gt.shape) > frac] = 0.0
-    return p.astype(np.float32)
+    spread = np.where(rng.uniform(size=gt.shape) < 0.5, 0.02 / gt, 0.0)
+    return p.astype(np.float32), spread.astype(np.float32)
 
 
 @pytest.mark.parametrize("down,rotated", [(1, 1), (1, 0), (0, 1), (0, 0)])
 def test_sweep_with_proposal_teacher_forced(tiny, gpu_ctx, down, rotated):
     spec, rp, sc, eng = tiny
     _say(f"case: sweep with proposal tiny down={down} rotated={rotated}")
-    prop = _proposal_plane(sc, seed=down * 2 + rotated)
-    pt = _plane(gpu_ctx, prop)
+    prop, spread = _proposal_plane(sc, seed=down * 2 + rotated)
+    pt, st = _plane(gpu_ctx, prop), _plane(gpu_ctx, spread)
     eng.prepare()
-    eng.set_proposal(pt.ptr, eng.W * 4)
+    eng.set_proposal(pt.ptr, eng.W * 4, st.ptr)
     eng.cost_all_views(rotated=True)
     eng.sweep(1, 1, 1)
     depth, costs, states = eng.depth(), eng.costs(), eng.rng()
     seeded = prop > 0
     S = rp.num_samples
     pb = oracle_for(eng, rp, sc)
-    pb.set_proposal(prop.astype(np.float64))
+    pb.set_proposal(prop.astype(np.float64), spread.astype(np.float64))
+    pb.set_sweep_id(1)   # the engine has run one sweep since prepare
     d_s = to_frame(depth.astype(np.float64), rotated)
     c_s = to_frame(costs.astype(np.float64), rotated)
     st_o = states.copy()
@@ -137,7 +139,7 @@ def test_set_proposal_seeds_depth(tiny, gpu_ctx):
     _say("case: proposal seeds the initial depth")
     eng.prepare()
     d0 = eng.depth()
-    prop = _proposal_plane(sc, seed=5)
+    prop, _ = _proposal_plane(sc, seed=5)
     pt = _plane(gpu_ctx, prop)
     eng.set_proposal(pt.ptr, eng.W * 4)
     d1 = eng.depth()
