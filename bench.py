@@ -88,6 +88,119 @@ def run_gpu_steps(eng, steps, warmup):
     return (t1 - t0), stats
 
 
+def proposal_planes(ctx, eng, net, tau):
+    """The network's proposal for the engine's (prepared) view, as host
+    planes: depth (0 = none) and spread in inverse depth, W x H."""
+    from patchmatch_cuda_amd import capi, proposal
+
+    W, H = eng.W, eng.H
+    W0, H0 = proposal.low_res(W, H)
+    x = capi.DeviceArray(ctx, (1, H0, W0, proposal.IN_C), np.uint16)
+    d = capi.DeviceArray(ctx, (H, W), np.float32)
+    sp = capi.DeviceArray(ctx, (H, W), np.float32)
+    eng.cost_volume(*eng.widest_views(), x.ptr)
+    net.forward(x)
+    net.propose(W, H, tau, d, sp)
+    out = d.download(), sp.download()
+    for a in (x, d, sp):
+        a.close()
+    return out
+
+
+def learned_leg(ctx, eng, sc, rp, weights, plain_gt, steps, warmup, tau):
+    """Learned-proposal PatchMatch on the same view: the smallest iteration
+    count whose depth is at least as accurate (share within 1% of the
+    rendered ground truth) as the plain method's full run, then timed at
+    that count.  A step = prepare + cost volume + network + proposal + run."""
+    from patchmatch_cuda_amd import proposal
+
+    W0, H0 = proposal.low_res(eng.W, eng.H)
+    net = proposal.Net(ctx, 1, H0, W0, rp.depth_near, rp.depth_far)
+    net.load(weights)
+    acc = {}
+    chosen = None
+    for k in range(1, rp.num_iterations + 1):
+        eng.set_num_iterations(k)
+        eng.prepare()
+        eng.propose(net, tau=tau)
+        eng.run()
+        acc[k] = float(np.mean(np.abs(eng.depth() - sc.gt_depth) <= 0.01 * sc.gt_depth))
+        log(f"learned, {k} iteration(s): {acc[k]:.4f} within 1% of ground truth "
+            f"(plain, {rp.num_iterations}: {plain_gt:.4f})")
+        if chosen is None and acc[k] >= plain_gt:
+            chosen = k
+            break
+    k = chosen or rp.num_iterations
+    eng.set_num_iterations(k)
+    for _ in range(warmup):
+        eng.prepare()
+        eng.propose(net, tau=tau)
+        eng.run()
+    ctx.sync()
+    eng.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.prepare()
+        eng.propose(net, tau=tau)
+        eng.run()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    eng.set_num_iterations(rp.num_iterations)
+    log(f"learned: {k} iteration(s), {wall * 1e3 / steps:.2f} ms/step")
+    return net, {
+        "iterations": k,
+        "sweeps": 4 * k,
+        "reached_plain_accuracy": chosen is not None,
+        "within_1pct_of_ground_truth": acc,
+        "ms_per_step": wall * 1e3 / steps,
+        "value": eng.W * eng.H * steps / wall / 1e6,
+        "tau": tau,
+        "weights": os.path.relpath(weights, ROOT),
+    }
+
+
+def learned_parity(ctx, sc, rp, sample, net, tau, k, full_eng):
+    """The learned GPU path on a crop against the FP64 oracle running the
+    same algorithm (same proposal planes, generator states and initial
+    depth): share of final depths within 1%."""
+    from oracle import pmx_oracle as O
+    from patchmatch_cuda_amd import capi, scene
+
+    W, H = sc.spec.W, sc.spec.H
+    w, h = sample
+    x0, y0 = (W - w) // 2, (H - h) // 2
+    full_eng.prepare()
+    prop, spread = proposal_planes(ctx, full_eng, net, tau)
+    prop = np.ascontiguousarray(prop[y0:y0 + h, x0:x0 + w])
+    spread = np.ascontiguousarray(spread[y0:y0 + h, x0:x0 + w])
+    ref, srcs, cams, gt = scene.crop_reference(sc, x0, y0, w, h)
+    eng = capi.Engine(ctx, rp.replace(num_iterations=k), ref, list(srcs), cams)
+    pd = capi.DeviceArray(ctx, (h, w), np.float32).upload(prop)
+    sd = capi.DeviceArray(ctx, (h, w), np.float32).upload(spread)
+    eng.prepare()
+    eng.set_proposal(pd.ptr, w * 4, sd.ptr)
+    d0, st0 = eng.depth(), eng.rng()
+    eng.run()
+    dg = eng.depth().astype(np.float64)
+    ref_b, srcs_b = eng.images()
+    f = lambda x: float(np.float32(x))  # noqa: E731
+    pb = O.Problem(ref_b, srcs_b, eng.homographies(), rp.half_window, f(rp.depth_near),
+                   f(rp.depth_far), f(rp.spm_alpha), O.inv_k4(cams[0, :9]),
+                   f(rp.orientation_x), f(rp.orientation_z))
+    pb.set_proposal(prop.astype(np.float64), spread.astype(np.float64))
+    do, _, _ = pb.run(d0.astype(np.float64), st0.copy(), rp.num_samples, k, record_ids=False)
+    eng.close()
+    pd.close()
+    sd.close()
+    agree = float(np.mean(np.abs(dg - do) <= 0.01 * np.abs(do)))
+    log(f"learned on crop: {agree:.4f} of depths within 1% of the oracle")
+    return {
+        "within_1pct_of_fp64_oracle": agree,
+        "sample": f"{w}x{h} centred crop, {k} iteration(s), oracle given the same proposal",
+        "gpu_within_1pct_of_ground_truth": float(np.mean(np.abs(dg - gt) <= 0.01 * gt)),
+    }
+
+
 def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads):
     """Oracle (FP64, OpenMP) and the GPU engine on the same reference crop."""
     from oracle import pmx_oracle as O
@@ -153,6 +266,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-waves", type=int, default=0,
                     help="wavefronts per sweep column (1, 2, 4; 0 = engine default)")
+    ap.add_argument("--weights", default=None,
+                    help="proposal network weights (default: the packaged file)")
+    ap.add_argument("--tau", type=float, default=0.0,
+                    help="confidence threshold of the learned proposal")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--fp16", action="store_true", help="FP16 texel storage")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -160,7 +279,7 @@ def main():
               "(a reference view is processed by a single device)", file=sys.stderr)
         sys.exit(2)
 
-    from patchmatch_cuda_amd import capi, configs, scene
+    from patchmatch_cuda_amd import capi, configs, proposal, scene
 
     spec, rp = configs.WORKLOADS[args.workload]
     t0 = time.perf_counter()
@@ -171,7 +290,11 @@ def main():
     eng = capi.Engine(ctx, rp, sc.ref, list(sc.srcs), sc.cams)
     if args.sweep_waves:
         eng.set_sweep_waves(args.sweep_waves)
+    if args.fp16:
+        eng.set_storage(capi.STORE_F16)
     wall, st = run_gpu_steps(eng, args.steps, args.warmup)
+    plain_depth = eng.depth()
+    plain_gt = float(np.mean(np.abs(plain_depth - sc.gt_depth) <= 0.01 * sc.gt_depth))
     npx = spec.W * spec.H
     ms_step = wall * 1e3 / args.steps
     value = npx * args.steps / wall / 1e6
@@ -192,7 +315,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f16 texels, f32 sums" if args.fp16 else "f32",
         "data": "synthetic (seeded piecewise-planar scene renderer, no dataset)",
         "config": {
             "workload": args.workload,
@@ -225,11 +348,29 @@ def main():
             "prepare": st["ms_prepare"] / args.steps,
         },
     }
+    line["plain"] = {"value": value, "ms_per_step": ms_step, "sweeps": 4 * rp.num_iterations,
+                     "within_1pct_of_ground_truth": plain_gt}
+    weights = args.weights or proposal.DEFAULT_WEIGHTS
+    net = None
+    if not args.no_learned and os.path.exists(weights):
+        net, lr = learned_leg(ctx, eng, sc, rp, weights, plain_gt, args.steps, args.warmup,
+                              args.tau)
+        lr["speedup_vs_plain"] = lr["value"] / value
+        lr["sweep_reduction_vs_plain"] = (4 * rp.num_iterations) / lr["sweeps"]
+        line["learned"] = lr
+    elif not args.no_learned:
+        line["learned"] = {"skipped": f"no weights at {os.path.relpath(weights, ROOT)}"}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         extra = cpu_baseline_and_accuracy(sc, rp, parse_sample(args.cpu_sample), ctx, True,
                                           threads)
         line.update(extra)
+        if net is not None:
+            line["learned"]["accuracy"] = learned_parity(
+                ctx, sc, rp, parse_sample(args.cpu_sample), net, args.tau,
+                line["learned"]["iterations"], eng)
+    if net is not None:
+        net.close()
     eng.close()
     ctx.close()
     print(json.dumps(line), flush=True)

@@ -37,6 +37,9 @@ extern "C" {
 #define PMX_IMG_F32 0         /* float32 intensities */
 #define PMX_IMG_U8 1          /* uchar, read as u/255 (cudaReadModeNormalizedFloat) */
 
+#define PMX_STORE_F32 0       /* source and reference texels stored as float32 */
+#define PMX_STORE_F16 1       /* stored as FP16; NCC sums still accumulate in FP32 */
+
 #define PMX_MEMCPY_H2D 1
 #define PMX_MEMCPY_D2H 2
 #define PMX_MEMCPY_D2D 3
@@ -211,6 +214,16 @@ int pmx_engine_set_sweep_waves(pmx_engine* eng, int waves);
 int pmx_engine_get_sweep_waves(pmx_engine* eng, int* waves);
 int pmx_engine_get_stats(pmx_engine* eng, pmx_stats* out);
 int pmx_engine_reset_stats(pmx_engine* eng);
+
+/* Texel storage of the pre-blurred images the NCC reads (PMX_STORE_*).  No
+ * reference counterpart: the reference samples uchar textures normalised to
+ * float (src/patchMatch.cu:27-28, :192-200).  PMX_STORE_F16 rounds the
+ * blurred sources and reference to the nearest FP16 value (ties to even)
+ * and halves the bytes of every bilinear tap; homographies stay FP64 and
+ * window sums FP32.  pmx_engine_get_images returns the rounded values.
+ * Takes effect at the next pmx_engine_prepare, which it requires. */
+int pmx_engine_set_storage(pmx_engine* eng, int storage);
+int pmx_engine_get_storage(pmx_engine* eng, int* storage);
 
 /* -------------------------------------------------- learned proposal */
 /* The reference has no learned component.  A proposal is one extra depth

@@ -18,6 +18,7 @@ PMX_OK = 0
 ERR_NAMES = {1: "PMX_ERR_ARG", 2: "PMX_ERR_HIP", 3: "PMX_ERR_ALLOC", 4: "PMX_ERR_STATE",
              5: "PMX_ERR_NO_DEVICE"}
 IMG_F32, IMG_U8 = 0, 1
+STORE_F32, STORE_F16 = 0, 1
 MEMCPY_H2D, MEMCPY_D2H, MEMCPY_D2D = 1, 2, 3
 
 RNG_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("z", "<u4"), ("w", "<u4"),
@@ -127,6 +128,8 @@ SIGNATURES = {
     "pmx_engine_set_num_iterations": [_vp, _i],
     "pmx_engine_set_proposal_seeding": [_vp, _i],
     "pmx_engine_propose": [_vp, _vp, _i, _i, _f],
+    "pmx_engine_set_storage": [_vp, _i],
+    "pmx_engine_get_storage": [_vp, ctypes.POINTER(_i)],
     "pmx_net_create": [_vp, _i, _i, _i, _f, _f, ctypes.c_ulonglong, ctypes.POINTER(_vp)],
     "pmx_net_destroy": [_vp],
     "pmx_net_num_params": [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
@@ -436,3 +439,12 @@ class Engine:
 
     def reset_stats(self):
         self._c(lib().pmx_engine_reset_stats(self.h))
+
+    def set_storage(self, storage):
+        """STORE_F32 or STORE_F16 texels for the NCC; needs a new prepare()."""
+        self._c(lib().pmx_engine_set_storage(self.h, int(storage)))
+
+    def storage(self):
+        v = ctypes.c_int()
+        self._c(lib().pmx_engine_get_storage(self.h, ctypes.byref(v)))
+        return v.value

@@ -152,8 +152,15 @@ struct Timer {
 PmxSweepArgs sweep_args(pmx_engine* e, int rotated) {
     PmxSweepArgs a{};
     a.src = e->src + (size_t)PMX_SRC_PAD * e->src_pitch + PMX_SRC_PAD;
-    a.src_block = e->src;
-    a.src_bytes = (long)sizeof(float) * e->src_layer * e->V;
+    if (e->storage == PMX_STORE_F16) {
+        a.src_block = e->src16;
+        a.src_bytes = (long)sizeof(uint16_t) * e->src_layer * e->V;
+        a.src_h16 = 1;
+    } else {
+        a.src_block = e->src;
+        a.src_bytes = (long)sizeof(float) * e->src_layer * e->V;
+        a.src_h16 = 0;
+    }
     a.src_layer = e->src_layer;
     a.src_pitch = e->src_pitch;
     a.srcW = e->maxW;
@@ -529,7 +536,7 @@ static void engine_free(pmx_engine* e) {
     void* bufs[] = {e->src_raw, e->src,     e->ref_raw,  e->ref,     e->sum_I,    e->sum_II,
                     e->stats,   e->Hm,      e->depth_rm, e->depth_cm, e->cost_rm, e->cost_cm,
                     e->spm,     e->rng,     e->used_ids, e->out_depth, e->out_cost, e->counters,
-                    e->prop_rm, e->prop_cm, e->spread_rm, e->spread_cm};
+                    e->prop_rm, e->prop_cm, e->spread_rm, e->spread_cm, e->src16};
     for (void* b : bufs) dev_free(b);
     for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
     delete e;
@@ -781,6 +788,13 @@ static int prepare_impl(pmx_engine* e, Timer* tm) {
             HIPCHK(ctx, pmx_launch_blur(s, layer, e->src_pitch, w, h, 1, 0, taps, n, tmp));
         }
     }
+    /* FP16 storage: the sources the NCC reads become FP16 (the FP32 block
+     * keeps the same rounded values for readback), and the reference is
+     * rounded the same way before its window sums */
+    if (e->storage == PMX_STORE_F16) {
+        HIPCHK(ctx, pmx_launch_quantize_f16(s, e->src, e->src16, (size_t)e->src_layer * V));
+        HIPCHK(ctx, pmx_launch_quantize_f16(s, e->ref, nullptr, (size_t)e->ref_pitch * H));
+    }
     /* reference box sums and NCC statistics (FP64) */
     double* dtmp = (double*)e->cost_cm; /* >= 2*W*H doubles when V >= 4; else allocate */
     void* own = nullptr;
@@ -863,6 +877,25 @@ int pmx_engine_widest_views(pmx_engine* e, int* view0, int* view1) {
     PMXCHK(check_engine(e));
     if (!view0 || !view1) return fail(e->ctx, PMX_ERR_ARG, "NULL argument");
     widest_views(e, view0, view1);
+    return PMX_OK;
+}
+
+int pmx_engine_set_storage(pmx_engine* e, int storage) {
+    PMXCHK(check_engine(e));
+    if (storage != PMX_STORE_F32 && storage != PMX_STORE_F16)
+        return fail(e->ctx, PMX_ERR_ARG, "unknown storage %d", storage);
+    if (storage == PMX_STORE_F16 && !e->src16)
+        PMXCHK(dev_alloc(e->ctx, (void**)&e->src16,
+                         sizeof(uint16_t) * (size_t)e->src_layer * e->V));
+    if (storage != e->storage) e->prepared = false;
+    e->storage = storage;
+    return PMX_OK;
+}
+
+int pmx_engine_get_storage(pmx_engine* e, int* storage) {
+    PMXCHK(check_engine(e));
+    if (!storage) return fail(e->ctx, PMX_ERR_ARG, "NULL argument");
+    *storage = e->storage;
     return PMX_OK;
 }
 

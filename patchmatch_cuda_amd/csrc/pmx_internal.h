@@ -41,10 +41,11 @@ struct pmx_ctx {
 /* Arguments of the sweep and cost kernels (device pointers). */
 struct PmxSweepArgs {
     const float* src;      /* view 0, texel (0,0) of the padded source block */
-    const float* src_block; /* start of the padded source block */
+    const void* src_block; /* start of the padded source block */
     long src_bytes;        /* its size (< 2^31: byte offsets are 32-bit) */
-    long src_layer;        /* floats between layers */
-    int src_pitch;         /* floats between rows */
+    long src_layer;        /* texels between layers */
+    int src_pitch;         /* texels between rows */
+    int src_h16;           /* 1: the block holds FP16 texels, 0: FP32 */
     int srcW, srcH;        /* packed source size */
     const float* ref;      /* pre-blurred reference, W x H */
     int ref_pitch;         /* floats */
@@ -83,6 +84,8 @@ struct pmx_engine {
     /* images */
     float* src_raw = nullptr;   /* padded source block as uploaded */
     float* src = nullptr;       /* padded source block, pre-blurred */
+    uint16_t* src16 = nullptr;  /* the same block as FP16 (storage PMX_STORE_F16) */
+    int storage = PMX_STORE_F32;
     long src_layer = 0;
     int src_pitch = 0;          /* floats */
     float* ref_raw = nullptr;   /* W x H, as uploaded */
@@ -162,6 +165,7 @@ hipError_t pmx_launch_cost_all(hipStream_t s, const PmxSweepArgs& a);
 hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a, int waves);
 hipError_t pmx_launch_cost_volume(hipStream_t s, const PmxSweepArgs& a, int view0, int view1,
                                   uint16_t* out, int W0, int H0);
+hipError_t pmx_launch_quantize_f16(hipStream_t s, float* data, uint16_t* out, size_t n);
 hipError_t pmx_launch_seed_depth(hipStream_t s, const float* prop, int prop_ld, float* depth,
                                  int depth_ld, int width, int height);
 hipError_t pmx_launch_gather_costs(hipStream_t s, const float* cost_rm, int cost_ld, float* out,

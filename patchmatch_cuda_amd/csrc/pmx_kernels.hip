@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "pmx_internal.h"
 
@@ -414,19 +415,26 @@ struct PixelCtx {
 };
 
 /* The padded source block as one raw buffer resource: 32-bit byte offsets,
- * descriptor in SGPRs. */
+ * descriptor in SGPRs.  H16: texels are FP16 (a bilinear row pair is one
+ * 4-byte load), else FP32 (one 8-byte load).  Tex is what a row load
+ * returns; it stays raw until the blend, so the conversion does not wait
+ * on the load inside the fetch batch. */
+template <bool H16>
 struct SrcBuf {
+    using Tex = typename std::conditional<H16, uint32_t, float2>::type;
+    static constexpr uint32_t kShift = H16 ? 1u : 2u; /* log2 bytes per texel */
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t layer_bytes, pitch_bytes;
     int srcW, srcH;
 };
 
-__device__ __forceinline__ SrcBuf src_buf(const PmxSweepArgs& a) {
-    SrcBuf b;
+template <bool H16>
+__device__ __forceinline__ SrcBuf<H16> src_buf(const PmxSweepArgs& a) {
+    SrcBuf<H16> b;
     b.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.src_block, (short)0, (int)a.src_bytes,
                                                0x00020000);
-    b.layer_bytes = (uint32_t)a.src_layer * 4u;
-    b.pitch_bytes = (uint32_t)a.src_pitch * 4u;
+    b.layer_bytes = (uint32_t)a.src_layer << SrcBuf<H16>::kShift;
+    b.pitch_bytes = (uint32_t)a.src_pitch << SrcBuf<H16>::kShift;
     b.srcW = a.srcW;
     b.srcH = a.srcH;
     return b;
@@ -464,30 +472,48 @@ struct TapW {
 };
 
 /* A tap in two halves, so that many taps' loads can be in flight before the
- * first blend: the two 8-byte loads (texel pairs of source rows floor(sy)
- * and floor(sy)+1) ... */
-__device__ __forceinline__ TapW tap_fetch(const SrcBuf& sb, const Hyp& q, float sx, float sy,
-                                          float2& t0, float2& t1) {
+ * first blend: the two row loads (texel pairs of source rows floor(sy) and
+ * floor(sy)+1) ... */
+template <class SB>
+__device__ __forceinline__ TapW tap_fetch(const SB& sb, const Hyp& q, float sx, float sy,
+                                          typename SB::Tex& t0, typename SB::Tex& t1) {
     sx = __builtin_amdgcn_fmed3f(sx, q.lox, q.hix);
     sy = __builtin_amdgcn_fmed3f(sy, q.loy, q.hiy);
     const int ix = floor_i(sx), iy = floor_i(sy);
     /* |iy| < 2^17 and the pitch < 2^17: a 24-bit multiply is exact */
-    const uint32_t off = (uint32_t)(__mul24(iy, (int)sb.pitch_bytes) + (int)q.base) + ((uint32_t)ix << 2);
-    t0 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, 0, 0));
-    t1 = __builtin_bit_cast(float2,
-                            __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, sb.pitch_bytes, 0));
+    const uint32_t off =
+        (uint32_t)(__mul24(iy, (int)sb.pitch_bytes) + (int)q.base) + ((uint32_t)ix << SB::kShift);
+    if constexpr (SB::kShift == 1) {
+        t0 = __builtin_amdgcn_raw_buffer_load_b32(sb.rsrc, off, 0, 0);
+        t1 = __builtin_amdgcn_raw_buffer_load_b32(sb.rsrc, off, sb.pitch_bytes, 0);
+    } else {
+        t0 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, 0, 0));
+        t1 = __builtin_bit_cast(float2,
+                                __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, sb.pitch_bytes, 0));
+    }
     return TapW{__builtin_amdgcn_fractf(sx), __builtin_amdgcn_fractf(sy)};
 }
 
-/* ... and their bilinear blend. */
-__device__ __forceinline__ float tap_blend(const TapW& w, float2 t0, float2 t1) {
+/* Two FP16 texels (low half first) widened to FP32, exactly. */
+__device__ __forceinline__ float2 tex_f32(uint32_t w) {
+    const _Float16 lo = __builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+    const _Float16 hi = __builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+    return make_float2((float)lo, (float)hi);
+}
+__device__ __forceinline__ float2 tex_f32(float2 t) { return t; }
+
+/* ... and their bilinear blend, in FP32. */
+template <class T>
+__device__ __forceinline__ float tap_blend(const TapW& w, T r0, T r1) {
+    const float2 t0 = tex_f32(r0), t1 = tex_f32(r1);
     const float top = fmaf(w.al, t0.y - t0.x, t0.x);
     const float bot = fmaf(w.al, t1.y - t1.x, t1.x);
     return fmaf(w.be, bot - top, top);
 }
 
-__device__ __forceinline__ float tap(const SrcBuf& sb, const Hyp& q, float sx, float sy) {
-    float2 t0, t1;
+template <class SB>
+__device__ __forceinline__ float tap(const SB& sb, const Hyp& q, float sx, float sy) {
+    typename SB::Tex t0, t1;
     const TapW w = tap_fetch(sb, q, sx, sy, t0, t1);
     return tap_blend(w, t0, t1);
 }
@@ -502,7 +528,8 @@ __device__ __forceinline__ double rcp64(double x) {
 }
 
 /* Hm: the V x 18 homographies (H1, H2), a copy in LDS */
-__device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const SrcBuf& sb, int view,
+template <class SB>
+__device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const SB& sb, int view,
                                         float depth, const PixelCtx& pc) {
     const double* __restrict__ Hv = Hm + view * 18;
     const double inv = rcp64((double)depth * pc.scale);
@@ -533,7 +560,7 @@ __device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const Src
     q.cx = (float)(h[6] * iz);
     q.cy = (float)(h[7] * iz);
     q.base = (uint32_t)view * sb.layer_bytes + (uint32_t)(iyc + PMX_SRC_PAD) * sb.pitch_bytes +
-             (uint32_t)(ixc + PMX_SRC_PAD) * 4u;
+             ((uint32_t)(ixc + PMX_SRC_PAD) << SB::kShift);
     return q;
 }
 
@@ -544,8 +571,8 @@ __device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const Src
  * taps of the column can be in flight at once: rows i >= nr still fetch a
  * (clamped, in-bounds) tap and read a zero padding row of the LDS window,
  * and their e is selected to 0. */
-template <int R, int NB = R>
-__device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q, int Wn, float dxf,
+template <int R, int NB = R, class SB>
+__device__ __forceinline__ void accum_rows(const SB& sb, const Hyp& q, int Wn, float dxf,
                                            float dy0, const float* __restrict__ wp, int nr,
                                            bool valid, float& K, float& A, float& B, float& C) {
     const float nxa = fmaf(q.uy, dy0, q.ux * dxf);
@@ -566,12 +593,12 @@ __device__ __forceinline__ void accum_rows(const SrcBuf& sb, const Hyp& q, int W
          * to its use, which would make the column R dependent round trips
          * to memory */
         /* the centre sample K goes out with the first batch */
-        float2 k0, k1;
+        typename SB::Tex k0, k1;
         TapW kw;
 #pragma unroll
         for (int b0 = 0; b0 < R; b0 += NB) {
             constexpr int kB = NB;
-            float2 t0[kB], t1[kB];
+            typename SB::Tex t0[kB], t1[kB];
             TapW tw[kB];
             if (b0 == 0) kw = tap_fetch(sb, q, q.fxc, q.fyc, k0, k1);
 #pragma unroll
@@ -640,9 +667,9 @@ __device__ __forceinline__ float ncc_finish(float A, float B, float C, float K,
  * sums to part[t * NW + slice] and the caller combines them after a
  * barrier (combine_pass).  Returns k.
  */
-template <int G, int WN, int NW>
+template <int G, int WN, int NW, class SB>
 __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __restrict__ Hm,
-                                         const SrcBuf& sb, const float* __restrict__ win,
+                                         const SB& sb, const float* __restrict__ win,
                                          int wave, int lane, int nt, const int* tview,
                                          const float* tdep, float* tout, float4* part,
                                          const PixelCtx& pc) {
@@ -803,7 +830,7 @@ __device__ __forceinline__ void load_homographies(const PmxSweepArgs& a, double*
 
 /* computeAllCostGivenDepth (src/patchMatch.cu:775-816): one wavefront per
  * pixel, G-lane groups over views.  Writes the row-major state. */
-template <int G, int PF, int WN>
+template <int G, int PF, int WN, bool H16>
 __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
     extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
@@ -822,7 +849,7 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
     const int pid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int x = pid % a.W, y = pid / a.W;
     const float inv_wn = 1.0f / (float)Wn;
-    const SrcBuf sb = src_buf(a);
+    const SrcBuf<H16> sb = src_buf<H16>(a);
     PixelCtx pc;
     pixel_ctx(a, x, y, gl, pc);
     const float2 st = a.stats[(size_t)y * a.stats_pitch + x];
@@ -858,7 +885,7 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
  * planes uniform in inverse depth on two source views.  Output: bf16 NHWC,
  * 64 channels: mean 1 - NCC of the two views per plane (clamped to [0, 2]),
  * the reference intensity, zeros. */
-template <int G, int PF, int WN>
+template <int G, int PF, int WN, bool H16>
 __global__ __launch_bounds__(64) void pmx_volume_kernel(PmxSweepArgs a, int view0, int view1,
                                                        uint16_t* __restrict__ out, int W0, int H0) {
     extern __shared__ double smem_d[];
@@ -877,7 +904,7 @@ __global__ __launch_bounds__(64) void pmx_volume_kernel(PmxSweepArgs a, int view
     const int X = pid % W0, Y = pid / W0;
     const int x = min(4 * X + 2, a.W - 1), y = min(4 * Y + 2, a.H - 1);
     const float inv_wn = 1.0f / (float)Wn;
-    const SrcBuf sb = src_buf(a);
+    const SrcBuf<H16> sb = src_buf<H16>(a);
     PixelCtx pc;
     pixel_ctx(a, x, y, gl, pc);
     const float2 st = a.stats[(size_t)y * a.stats_pitch + x];
@@ -906,6 +933,16 @@ __global__ __launch_bounds__(64) void pmx_volume_kernel(PmxSweepArgs a, int view
     /* bf16, round to nearest even */
     const uint32_t u = __float_as_uint(v);
     out[((size_t)Y * W0 + X) * 64 + lane] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+/* FP16 storage: rounds n floats to the nearest FP16 value (ties to even)
+ * in place and, when out is not null, stores the FP16 bits there. */
+__global__ void quantize_f16_kernel(float* __restrict__ data, uint16_t* __restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const _Float16 h = (_Float16)data[i];
+    data[i] = (float)h;
+    if (out) out[i] = __builtin_bit_cast(uint16_t, h);
 }
 
 /* depth = prop where prop > 0 (both in the same layout) */
@@ -1012,7 +1049,7 @@ __device__ __forceinline__ float proposal_draw(float depth, float spread, int x,
  * (the reference recomputes the same value).  Wave 0 alone writes the
  * column's outputs.
  */
-template <int G, int PF, int WN, int NW>
+template <int G, int PF, int WN, int NW, bool H16>
 /* A 2048-column sweep puts 8 workgroups on each of the 256 CUs: with NW
  * waves each that is 2*NW waves per SIMD, all of which must be resident at
  * once (a column that waits for a free slot doubles the sweep time), so
@@ -1043,7 +1080,7 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
     const bool is_view = lane < V;
     const int vl = is_view ? lane : 0;
     const float inv_wn = 1.0f / (float)Wn;
-    const SrcBuf sb = src_buf(a);
+    const SrcBuf<H16> sb = src_buf<H16>(a);
     const MsgConst mc = msg_const(a.alpha2);
 
     float* dcol = a.depth + (size_t)c * a.depth_ld;
@@ -1389,8 +1426,11 @@ template <int G, int PF, int WN>
 hipError_t launch_cost_all_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
     const size_t sh = sizeof(double) * a.V * 18 + sizeof(float) * (2 * Wn * Wn + 3 * 64);
-    hipLaunchKernelGGL((pmx_cost_all_kernel<G, PF, WN>), dim3((unsigned)((long)a.W * a.H)), dim3(64),
-                       sh, s, a);
+    const dim3 grid((unsigned)((long)a.W * a.H));
+    if (a.src_h16)
+        hipLaunchKernelGGL((pmx_cost_all_kernel<G, PF, WN, true>), grid, dim3(64), sh, s, a);
+    else
+        hipLaunchKernelGGL((pmx_cost_all_kernel<G, PF, WN, false>), grid, dim3(64), sh, s, a);
     return hipGetLastError();
 }
 
@@ -1399,8 +1439,13 @@ hipError_t launch_volume_t(hipStream_t s, const PmxSweepArgs& a, int v0, int v1,
                            int W0, int H0) {
     const int Wn = 2 * a.hw + 1;
     const size_t sh = sizeof(double) * a.V * 18 + sizeof(float) * (2 * Wn * Wn + 3 * 64);
-    hipLaunchKernelGGL((pmx_volume_kernel<G, PF, WN>), dim3((unsigned)((long)W0 * H0)), dim3(64),
-                       sh, s, a, v0, v1, out, W0, H0);
+    const dim3 grid((unsigned)((long)W0 * H0));
+    if (a.src_h16)
+        hipLaunchKernelGGL((pmx_volume_kernel<G, PF, WN, true>), grid, dim3(64), sh, s, a, v0, v1,
+                           out, W0, H0);
+    else
+        hipLaunchKernelGGL((pmx_volume_kernel<G, PF, WN, false>), grid, dim3(64), sh, s, a, v0, v1,
+                           out, W0, H0);
     return hipGetLastError();
 }
 
@@ -1409,7 +1454,11 @@ hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
     const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * 2 * 64 * NW +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
-    hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW>), dim3(a.Ws), dim3(64 * NW), sh, s, a);
+    if (a.src_h16)
+        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, true>), dim3(a.Ws), dim3(64 * NW), sh, s, a);
+    else
+        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, false>), dim3(a.Ws), dim3(64 * NW), sh, s,
+                           a);
     return hipGetLastError();
 }
 } // namespace
@@ -1434,6 +1483,13 @@ hipError_t pmx_launch_cost_volume(hipStream_t s, const PmxSweepArgs& a, int view
     if (Wn <= 16) return launch_volume_t<16, 4, 0>(s, a, view0, view1, out, W0, H0);
     if (Wn <= 32) return launch_volume_t<32, 16, 0>(s, a, view0, view1, out, W0, H0);
     return launch_volume_t<64, 44, 0>(s, a, view0, view1, out, W0, H0);
+}
+
+hipError_t pmx_launch_quantize_f16(hipStream_t s, float* data, uint16_t* out, size_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(quantize_f16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, data,
+                       out, n);
+    return hipGetLastError();
 }
 
 hipError_t pmx_launch_seed_depth(hipStream_t s, const float* prop, int prop_ld, float* depth,

@@ -48,7 +48,7 @@ PLANES = 32
 
 _vp = ctypes.c_void_p
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "proposal_bf16.npy")
+DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "proposal_bf16.pmxw")
 
 
 def low_res(W, H):
@@ -212,7 +212,13 @@ def make_dataset(ctx, n, W=640, H=480, V=6, seed0=10_000, threads=16, log=print)
     return x_all, t_all
 
 
-def train(net, x_all, t_all, epochs, batch=8, lr=1e-3, w_conf=0.1, seed=0, log=print):
+def train(net, x_all, t_all, epochs, batch=8, lr=1e-3, w_conf=0.1, seed=0, log=print,
+          warmup_steps=500):
+    """Adam at learning rate lr after a linear warm-up over warmup_steps
+    steps.  Without it the first, sign-like Adam steps move every weight of
+    this normalisation-free network by lr at once, which scales the
+    activations of 15 stacked convolutions by orders of magnitude and
+    saturates the head's sigmoids (gradients then vanish for good)."""
     n = x_all.shape[0]
     rng = np.random.default_rng(seed)
     xb = capi.DeviceArray(net.ctx, (batch,) + tuple(x_all.shape[1:]), x_all.dtype)
@@ -227,12 +233,14 @@ def train(net, x_all, t_all, epochs, batch=8, lr=1e-3, w_conf=0.1, seed=0, log=p
             for j, k in enumerate(order[s:s + batch]):
                 xb.copy_rows(j, x_all, int(k))
                 tb.copy_rows(j, t_all, int(k))
-            loss = net.train_step(xb, tb, lr, w_conf)
+            it = ep * (n // batch) + steps
+            lr_t = lr * min(1.0, (it + 1) / warmup_steps) if warmup_steps > 0 else lr
+            loss = net.train_step(xb, tb, lr_t, w_conf)
             sums += loss
             steps += 1
         hist.append((sums / max(steps, 1)).tolist())
-        log(f"epoch {ep + 1}/{epochs}: L1 {hist[-1][0]:.5f} BCE {hist[-1][1]:.4f} "
-            f"({time.time() - t0:.1f} s)")
+        log(f"epoch {ep + 1}/{epochs}: L1 on inverse depth {hist[-1][0]:.5f}, "
+            f"L1 of the error scale {hist[-1][1]:.5f} ({time.time() - t0:.1f} s)")
     return hist
 
 
@@ -244,6 +252,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--warmup-steps", type=int, default=500)
     a = ap.parse_args(argv)
     ctx = capi.Context(0)
     W, H, V = 640, 480, 6
@@ -252,7 +261,8 @@ def main(argv=None):
     x_all, t_all = make_dataset(ctx, a.scenes, W, H, V, log=log)
     _, rp = configs.C2_REALISTIC
     net = Net(ctx, a.batch, H0, W0, rp.depth_near, rp.depth_far, seed=a.seed)
-    hist = train(net, x_all, t_all, a.epochs, a.batch, a.lr, seed=a.seed, log=log)
+    hist = train(net, x_all, t_all, a.epochs, a.batch, a.lr, seed=a.seed, log=log,
+                 warmup_steps=a.warmup_steps)
     net.save(a.out)
     log(json.dumps({"weights": a.out, "scenes": a.scenes, "epochs": a.epochs, "loss": hist}))
     return 0
