@@ -107,37 +107,91 @@ def proposal_planes(ctx, eng, net, tau):
     return out
 
 
-def learned_leg(ctx, eng, sc, rp, weights, plain_gt, steps, warmup, tau):
-    """Learned-proposal PatchMatch on the same view: the smallest iteration
-    count whose depth is at least as accurate (share within 1% of the
-    rendered ground truth) as the plain method's full run, then timed at
-    that count.  A step = prepare + cost volume + network + proposal + run."""
-    from patchmatch_cuda_amd import proposal
+def crop_problem(sc, sample):
+    """The centred crop of the reference view (all sources kept)."""
+    from patchmatch_cuda_amd import scene
+
+    W, H = sc.spec.W, sc.spec.H
+    w, h = sample
+    x0, y0 = (W - w) // 2, (H - h) // 2
+    return (x0, y0, w, h), scene.crop_reference(sc, x0, y0, w, h)
+
+
+def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau):
+    """Learned-proposal PatchMatch against plain PatchMatch.
+
+    Sweeps needed: on the centred crop, each method runs 1..iterations
+    iterations and its final depth is compared with the converged FP64
+    oracle (plain method, all iterations, d_oracle); a method's iteration
+    count is the smallest whose depth has >= 99% of pixels within 1%.
+    The learned method is then timed on the full view at its count (a step
+    = prepare + cost volume + network + proposal + run), and its parity is
+    checked: at that count, the GPU's learned run on the crop against the
+    FP64 oracle running the same learned algorithm (same proposal planes,
+    generator states and initial depth)."""
+    from oracle import pmx_oracle as O
+    from patchmatch_cuda_amd import capi, proposal
 
     W0, H0 = proposal.low_res(eng.W, eng.H)
     net = proposal.Net(ctx, 1, H0, W0, rp.depth_near, rp.depth_far)
     net.load(weights)
-    acc = {}
-    chosen = None
+    (x0, y0, w, h), (ref, srcs, cams, gt) = crop_problem(sc, sample)
+    eng.prepare()
+    prop, spread = proposal_planes(ctx, eng, net, tau)
+    prop = np.ascontiguousarray(prop[y0:y0 + h, x0:x0 + w])
+    spread = np.ascontiguousarray(spread[y0:y0 + h, x0:x0 + w])
+    ceng = capi.Engine(ctx, rp, ref, list(srcs), cams)
+    pd = capi.DeviceArray(ctx, (h, w), np.float32).upload(prop)
+    sd = capi.DeviceArray(ctx, (h, w), np.float32).upload(spread)
+    agree = lambda d: float(np.mean(np.abs(d - d_oracle) <= 0.01 * np.abs(d_oracle)))  # noqa
+    on_gt = lambda d: float(np.mean(np.abs(d - gt) <= 0.01 * gt))  # noqa: E731
+    study = {"plain": {}, "learned": {}}
+    need = {"plain": None, "learned": None}
     for k in range(1, rp.num_iterations + 1):
-        eng.set_num_iterations(k)
-        eng.prepare()
-        eng.propose(net, tau=tau)
-        eng.run()
-        acc[k] = float(np.mean(np.abs(eng.depth() - sc.gt_depth) <= 0.01 * sc.gt_depth))
-        log(f"learned, {k} iteration(s): {acc[k]:.4f} within 1% of ground truth "
-            f"(plain, {rp.num_iterations}: {plain_gt:.4f})")
-        if chosen is None and acc[k] >= plain_gt:
-            chosen = k
-            break
-    k = chosen or rp.num_iterations
+        ceng.set_num_iterations(k)
+        for method in ("plain", "learned"):
+            ceng.prepare()
+            if method == "learned":
+                ceng.set_proposal(pd.ptr, w * 4, sd.ptr)
+            ceng.run()
+            d = ceng.depth().astype(np.float64)
+            ceng.set_proposal(None, 0)
+            a = agree(d)
+            study[method][k] = {"within_1pct_of_converged_oracle": a,
+                                "within_1pct_of_ground_truth": on_gt(d)}
+            if need[method] is None and a >= 0.99:
+                need[method] = k
+        log(f"crop, {k} iteration(s): plain {study['plain'][k]['within_1pct_of_converged_oracle']:.4f}"
+            f", learned {study['learned'][k]['within_1pct_of_converged_oracle']:.4f} of depths "
+            f"within 1% of the converged oracle")
+    k = need["learned"] or rp.num_iterations
+    # parity of the learned path at k
+    ceng.set_num_iterations(k)
+    ceng.prepare()
+    ceng.set_proposal(pd.ptr, w * 4, sd.ptr)
+    d0, st0 = ceng.depth(), ceng.rng()
+    ceng.run()
+    dg = ceng.depth().astype(np.float64)
+    ref_b, srcs_b = ceng.images()
+    f = lambda x: float(np.float32(x))  # noqa: E731
+    pb = O.Problem(ref_b, srcs_b, ceng.homographies(), rp.half_window, f(rp.depth_near),
+                   f(rp.depth_far), f(rp.spm_alpha), O.inv_k4(cams[0, :9]),
+                   f(rp.orientation_x), f(rp.orientation_z))
+    pb.set_proposal(prop.astype(np.float64), spread.astype(np.float64))
+    do, _, _ = pb.run(d0.astype(np.float64), st0.copy(), rp.num_samples, k, record_ids=False)
+    parity = float(np.mean(np.abs(dg - do) <= 0.01 * np.abs(do)))
+    log(f"learned on crop, {k} iteration(s): {parity:.4f} of depths within 1% of the oracle "
+        f"running the same learned algorithm")
+    ceng.close()
+    pd.close()
+    sd.close()
+    # full-view timing at k
     eng.set_num_iterations(k)
     for _ in range(warmup):
         eng.prepare()
         eng.propose(net, tau=tau)
         eng.run()
     ctx.sync()
-    eng.reset_stats()
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.prepare()
@@ -145,59 +199,153 @@ def learned_leg(ctx, eng, sc, rp, weights, plain_gt, steps, warmup, tau):
         eng.run()
     ctx.sync()
     wall = time.perf_counter() - t0
+    full_gt = float(np.mean(np.abs(eng.depth() - sc.gt_depth) <= 0.01 * sc.gt_depth))
+    eng.set_proposal(None, 0)
     eng.set_num_iterations(rp.num_iterations)
+    net.close()
     log(f"learned: {k} iteration(s), {wall * 1e3 / steps:.2f} ms/step")
-    return net, {
+    return {
         "iterations": k,
         "sweeps": 4 * k,
-        "reached_plain_accuracy": chosen is not None,
-        "within_1pct_of_ground_truth": acc,
         "ms_per_step": wall * 1e3 / steps,
         "value": eng.W * eng.H * steps / wall / 1e6,
+        "within_1pct_of_ground_truth": full_gt,
+        "iterations_to_99pct_of_converged_oracle": need,
+        "sweep_study": study,
+        "parity": {"within_1pct_of_fp64_oracle": parity,
+                   "sample": f"{w}x{h} centred crop, {k} iteration(s), oracle given the same "
+                             f"proposal planes"},
         "tau": tau,
         "weights": os.path.relpath(weights, ROOT),
     }
 
 
-def learned_parity(ctx, sc, rp, sample, net, tau, k, full_eng):
-    """The learned GPU path on a crop against the FP64 oracle running the
-    same algorithm (same proposal planes, generator states and initial
-    depth): share of final depths within 1%."""
-    from oracle import pmx_oracle as O
-    from patchmatch_cuda_amd import capi, scene
+def batch_bench(args):
+    """The batched workload: 16 reference views of mixed resolution, FP16
+    texels, one persistent sweep launch per sweep step over all views,
+    replayed from a hipGraph, launch shape from the autotuner.  Compared
+    with the same views run one at a time (FP32 and FP16).  Per-view error:
+    each view's centred crop, run in a batch of crops, against the FP64
+    oracle on the same FP16-rounded images (parity) and on the FP32 images
+    (the precision cost)."""
+    from concurrent.futures import ThreadPoolExecutor
 
-    W, H = sc.spec.W, sc.spec.H
-    w, h = sample
-    x0, y0 = (W - w) // 2, (H - h) // 2
-    full_eng.prepare()
-    prop, spread = proposal_planes(ctx, full_eng, net, tau)
-    prop = np.ascontiguousarray(prop[y0:y0 + h, x0:x0 + w])
-    spread = np.ascontiguousarray(spread[y0:y0 + h, x0:x0 + w])
-    ref, srcs, cams, gt = scene.crop_reference(sc, x0, y0, w, h)
-    eng = capi.Engine(ctx, rp.replace(num_iterations=k), ref, list(srcs), cams)
-    pd = capi.DeviceArray(ctx, (h, w), np.float32).upload(prop)
-    sd = capi.DeviceArray(ctx, (h, w), np.float32).upload(spread)
-    eng.prepare()
-    eng.set_proposal(pd.ptr, w * 4, sd.ptr)
-    d0, st0 = eng.depth(), eng.rng()
-    eng.run()
-    dg = eng.depth().astype(np.float64)
-    ref_b, srcs_b = eng.images()
+    from oracle import pmx_oracle as O
+    from patchmatch_cuda_amd import capi, configs, scene
+
+    views = configs.c3_batch()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        scenes = list(pool.map(lambda v: scene.render(v[0]), views))
+    log(f"rendered {len(views)} views in {time.perf_counter() - t0:.1f} s")
+    ctx = capi.Context(0)
+    engs = [capi.Engine(ctx, rp, sc.ref, list(sc.srcs), sc.cams) for (_, rp), sc in
+            zip(views, scenes)]
+    npx = sum(e.W * e.H for e in engs)
+
+    def one_at_a_time(storage):
+        for e in engs:
+            e.set_storage(storage)
+        for rep in range(2):             # warm-up pass, then the timed pass
+            ctx.sync()
+            t = time.perf_counter()
+            for e in engs:
+                e.prepare()
+                e.run()
+            ctx.sync()
+            dt = time.perf_counter() - t
+        return npx / dt / 1e6, dt * 1e3
+
+    seq32, seq32_ms = one_at_a_time(capi.STORE_F32)
+    log(f"one view at a time, FP32: {seq32:.2f} Mpix/s")
+    seq16, seq16_ms = one_at_a_time(capi.STORE_F16)
+    log(f"one view at a time, FP16: {seq16:.2f} Mpix/s")
+    b = capi.Batch(ctx, engs)
+    waves, wgs, tune_ms = b.autotune()
+    log(f"autotuned: {waves} wave(s) per column strip, {wgs} workgroups ({tune_ms:.1f} ms)")
+    b.set_graph(True)
+    for _ in range(args.warmup):
+        b.prepare()
+        b.run()
+    ctx.sync()
+    run_ms = 0.0
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        b.prepare()
+        run_ms += b.run(timed=True)
+    ctx.sync()
+    wall = time.perf_counter() - t
+    value = npx * args.steps / wall / 1e6
+    log(f"batched: {value:.2f} Mpix/s ({wall * 1e3 / args.steps:.1f} ms/step)")
+    b.close()
+    for e in engs:
+        e.close()
+
+    # per-view error on centred crops
+    cw, ch = parse_sample(args.batch_crop)
+    crops, cengs = [], []
+    for (spec, rp), sc in zip(views, scenes):
+        x0, y0 = (spec.W - cw) // 2, (spec.H - ch) // 2
+        ref, srcs, cams, gt = scene.crop_reference(sc, x0, y0, cw, ch)
+        e = capi.Engine(ctx, rp, ref, list(srcs), cams)
+        e.set_storage(capi.STORE_F16)
+        crops.append((ref, srcs, cams, gt, rp))
+        cengs.append(e)
+    cb = capi.Batch(ctx, cengs)
+    cb.set_graph(True)
+    cb.prepare()
+    init = [(e.depth(), e.rng()) for e in cengs]
+    cb.run()
+    O.set_threads(min(16, os.cpu_count() or 1))
     f = lambda x: float(np.float32(x))  # noqa: E731
-    pb = O.Problem(ref_b, srcs_b, eng.homographies(), rp.half_window, f(rp.depth_near),
-                   f(rp.depth_far), f(rp.spm_alpha), O.inv_k4(cams[0, :9]),
-                   f(rp.orientation_x), f(rp.orientation_z))
-    pb.set_proposal(prop.astype(np.float64), spread.astype(np.float64))
-    do, _, _ = pb.run(d0.astype(np.float64), st0.copy(), rp.num_samples, k, record_ids=False)
-    eng.close()
-    pd.close()
-    sd.close()
-    agree = float(np.mean(np.abs(dg - do) <= 0.01 * np.abs(do)))
-    log(f"learned on crop: {agree:.4f} of depths within 1% of the oracle")
+    per_view = []
+    for i, (e, (ref, srcs, cams, gt, rp)) in enumerate(zip(cengs, crops)):
+        dg = e.depth().astype(np.float64)
+        r16, s16 = e.images()
+        H = e.homographies()
+        res = {"view": i, "size": f"{views[i][0].W}x{views[i][0].H}"}
+        for tag, (ri, si) in (("rounded", (r16, s16)), ("fp32", (
+                O.blur(ref, rp.gaussian_sigma, rp.blur_mode),
+                np.stack([O.blur(s_, rp.gaussian_sigma, rp.blur_mode) for s_ in srcs])))):
+            pb = O.Problem(ri, si, H, rp.half_window, f(rp.depth_near), f(rp.depth_far),
+                           f(rp.spm_alpha), O.inv_k4(cams[0, :9]), f(rp.orientation_x),
+                           f(rp.orientation_z))
+            do, _, _ = pb.run(init[i][0].astype(np.float64), init[i][1].copy(), rp.num_samples,
+                              rp.num_iterations, record_ids=False)
+            res[f"within_1pct_of_oracle_{tag}"] = float(np.mean(np.abs(dg - do) <= 0.01 * np.abs(do)))
+            res[f"oracle_{tag}_within_1pct_of_ground_truth"] = float(
+                np.mean(np.abs(do - gt) <= 0.01 * gt))
+        res["gpu_within_1pct_of_ground_truth"] = float(np.mean(np.abs(dg - gt) <= 0.01 * gt))
+        per_view.append(res)
+        log(f"view {i} ({res['size']}): {res['within_1pct_of_oracle_rounded']:.4f} within 1% of the "
+            f"oracle on the FP16 images, {res['within_1pct_of_oracle_fp32']:.4f} of the oracle on "
+            f"FP32 images")
+    cb.close()
+    for e in cengs:
+        e.close()
+    ctx.close()
     return {
-        "within_1pct_of_fp64_oracle": agree,
-        "sample": f"{w}x{h} centred crop, {k} iteration(s), oracle given the same proposal",
-        "gpu_within_1pct_of_ground_truth": float(np.mean(np.abs(dg - gt) <= 0.01 * gt)),
+        "metric": "batched depth Mpix/s (16 views, 10 src views each, mixed resolution)",
+        "value": value,
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 texels, f32 sums, f64 homographies",
+        "data": "synthetic (16 seeded scenes, no dataset)",
+        "config": {"workload": "c3_batch", "views": [f"{e[0].W}x{e[0].H}" for e in views],
+                   "src_views": 10, "window": 15, "iterations": 4, "num_samples": 5,
+                   "graph": True, "waves_per_strip": waves, "workgroups": wgs},
+        "gpu_ms_per_run": run_ms / args.steps,
+        "one_view_at_a_time": {"fp32_mpix_s": seq32, "fp32_ms": seq32_ms,
+                               "fp16_mpix_s": seq16, "fp16_ms": seq16_ms},
+        "speedup_vs_one_at_a_time_fp32": value / seq32,
+        "per_view_error": {"sample": f"centred {cw}x{ch} crop of each view, batched, FP16",
+                           "views": per_view},
     }
 
 
@@ -250,7 +398,7 @@ def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads):
             "oracle_within_1pct_of_ground_truth": float(np.mean(np.abs(d_o - gt) <= 0.01 * gt)),
         }
         log(f"accuracy on crop: {agree:.4f} of depths within 1% of the oracle")
-    return out
+    return out, d_o
 
 
 def main():
@@ -268,16 +416,22 @@ def main():
                     help="wavefronts per sweep column (1, 2, 4; 0 = engine default)")
     ap.add_argument("--weights", default=None,
                     help="proposal network weights (default: the packaged file)")
-    ap.add_argument("--tau", type=float, default=0.0,
+    ap.add_argument("--tau", type=float, default=0.3,
                     help="confidence threshold of the learned proposal")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--fp16", action="store_true", help="FP16 texel storage")
+    ap.add_argument("--batch-crop", default="128x96",
+                    help="per-view crop of the batched workload's oracle check")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(a reference view is processed by a single device)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c3_batch":
+        print(json.dumps(batch_bench(args)), flush=True)
+        return
 
     from patchmatch_cuda_amd import capi, configs, proposal, scene
 
@@ -351,26 +505,24 @@ def main():
     line["plain"] = {"value": value, "ms_per_step": ms_step, "sweeps": 4 * rp.num_iterations,
                      "within_1pct_of_ground_truth": plain_gt}
     weights = args.weights or proposal.DEFAULT_WEIGHTS
-    net = None
-    if not args.no_learned and os.path.exists(weights):
-        net, lr = learned_leg(ctx, eng, sc, rp, weights, plain_gt, args.steps, args.warmup,
-                              args.tau)
+    d_oracle = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        extra, d_oracle = cpu_baseline_and_accuracy(sc, rp, parse_sample(args.cpu_sample), ctx,
+                                                    True, threads)
+        line.update(extra)
+    if args.no_learned:
+        pass
+    elif not os.path.exists(weights):
+        line["learned"] = {"skipped": f"no weights at {os.path.relpath(weights, ROOT)}"}
+    elif d_oracle is None:
+        line["learned"] = {"skipped": "needs the oracle crop (--no-cpu-baseline given)"}
+    else:
+        lr = learned_leg(ctx, eng, sc, rp, weights, d_oracle, parse_sample(args.cpu_sample),
+                         args.steps, args.warmup, args.tau)
         lr["speedup_vs_plain"] = lr["value"] / value
         lr["sweep_reduction_vs_plain"] = (4 * rp.num_iterations) / lr["sweeps"]
         line["learned"] = lr
-    elif not args.no_learned:
-        line["learned"] = {"skipped": f"no weights at {os.path.relpath(weights, ROOT)}"}
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        extra = cpu_baseline_and_accuracy(sc, rp, parse_sample(args.cpu_sample), ctx, True,
-                                          threads)
-        line.update(extra)
-        if net is not None:
-            line["learned"]["accuracy"] = learned_parity(
-                ctx, sc, rp, parse_sample(args.cpu_sample), net, args.tau,
-                line["learned"]["iterations"], eng)
-    if net is not None:
-        net.close()
     eng.close()
     ctx.close()
     print(json.dumps(line), flush=True)

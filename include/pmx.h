@@ -225,6 +225,38 @@ int pmx_engine_reset_stats(pmx_engine* eng);
 int pmx_engine_set_storage(pmx_engine* eng, int storage);
 int pmx_engine_get_storage(pmx_engine* eng, int* storage);
 
+/* ------------------------------------------------------------ batches */
+/* Many reference views (each its own engine: images, cameras, state) run
+ * the runPatchMatch schedule (src/patchMatch.cu:263-407) together.  The
+ * reference runs one refImageId per process (src/main.cpp:118-127); here
+ * each of the 4 x numOfIterations sweep steps is ONE persistent launch over
+ * the column strips of every view, longest columns first, and the whole
+ * schedule (cost of all views, sweeps, transposes) is captured once in a
+ * hipGraph and replayed.  The engines must share half_window, num_samples,
+ * num_iterations and storage; their sizes and view counts may differ.
+ * Each engine's results equal a pmx_engine_run of that engine alone with
+ * the same launch shape (waves per column strip). */
+typedef struct pmx_batch pmx_batch;
+int pmx_batch_create(pmx_ctx* ctx, pmx_engine* const* engines, int num_views, pmx_batch** out);
+int pmx_batch_destroy(pmx_batch* batch);
+/* pmx_engine_prepare of every engine. */
+int pmx_batch_prepare(pmx_batch* batch);
+/* The schedule of every view, on freshly prepared engines (PMX_ERR_STATE
+ * otherwise).  ms (optional): wall time of the run from HIP events. */
+int pmx_batch_run(pmx_batch* batch, float* ms);
+/* Replay from a hipGraph (default 1); the graph is rebuilt when the launch
+ * shape, storage, iterations or proposal presence change. */
+int pmx_batch_set_graph(pmx_batch* batch, int enabled);
+/* Launch shape: waves per column strip (1 or 2) and persistent workgroups
+ * (0 = every slot the device keeps resident, else at least 8: one per XCD
+ * queue). */
+int pmx_batch_set_launch(pmx_batch* batch, int waves, int workgroups);
+int pmx_batch_get_launch(pmx_batch* batch, int* waves, int* workgroups);
+/* Autotuner: times full runs (prepare + run) for waves 1 and 2 at all,
+ * 3/4 and 1/2 of the resident workgroups, keeps the fastest and returns it.
+ * The engines are left run; prepare them again before pmx_batch_run. */
+int pmx_batch_autotune(pmx_batch* batch, int* waves, int* workgroups, float* ms);
+
 /* -------------------------------------------------- learned proposal */
 /* The reference has no learned component.  A proposal is one extra depth
  * hypothesis per pixel per sweep, a fourth candidate beside the former,

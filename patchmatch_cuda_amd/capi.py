@@ -129,6 +129,14 @@ SIGNATURES = {
     "pmx_engine_set_proposal_seeding": [_vp, _i],
     "pmx_engine_propose": [_vp, _vp, _i, _i, _f],
     "pmx_engine_set_storage": [_vp, _i],
+    "pmx_batch_create": [_vp, _vp, _i, ctypes.POINTER(_vp)],
+    "pmx_batch_destroy": [_vp],
+    "pmx_batch_prepare": [_vp],
+    "pmx_batch_run": [_vp, ctypes.POINTER(_f)],
+    "pmx_batch_set_graph": [_vp, _i],
+    "pmx_batch_set_launch": [_vp, _i, _i],
+    "pmx_batch_get_launch": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "pmx_batch_autotune": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_f)],
     "pmx_engine_get_storage": [_vp, ctypes.POINTER(_i)],
     "pmx_net_create": [_vp, _i, _i, _i, _f, _f, ctypes.c_ulonglong, ctypes.POINTER(_vp)],
     "pmx_net_destroy": [_vp],
@@ -448,3 +456,51 @@ class Engine:
         v = ctypes.c_int()
         self._c(lib().pmx_engine_get_storage(self.h, ctypes.byref(v)))
         return v.value
+
+
+class Batch:
+    """Several engines run together (pmx_batch_*): one persistent sweep
+    launch per sweep step over every view, replayed from a hipGraph."""
+
+    def __init__(self, ctx: Context, engines):
+        self.ctx = ctx
+        self.engines = list(engines)
+        arr = (_vp * len(self.engines))(*[e.h for e in self.engines])
+        h = _vp()
+        ctx.check(lib().pmx_batch_create(ctx.h, arr, len(self.engines), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pmx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def prepare(self):
+        self.ctx.check(lib().pmx_batch_prepare(self.h))
+
+    def run(self, timed=False):
+        """Runs the schedule; returns its duration in ms when timed."""
+        ms = _f()
+        self.ctx.check(lib().pmx_batch_run(self.h, ctypes.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def set_graph(self, on=True):
+        self.ctx.check(lib().pmx_batch_set_graph(self.h, int(bool(on))))
+
+    def set_launch(self, waves, workgroups=0):
+        self.ctx.check(lib().pmx_batch_set_launch(self.h, int(waves), int(workgroups)))
+
+    def launch(self):
+        w, g = _i(), _i()
+        self.ctx.check(lib().pmx_batch_get_launch(self.h, ctypes.byref(w), ctypes.byref(g)))
+        return w.value, g.value
+
+    def autotune(self):
+        """(waves, workgroups, ms) of the fastest launch shape, now in use."""
+        w, g, ms = _i(), _i(), _f()
+        self.ctx.check(lib().pmx_batch_autotune(self.h, ctypes.byref(w), ctypes.byref(g),
+                                                ctypes.byref(ms)))
+        return w.value, g.value, ms.value

@@ -1199,3 +1199,365 @@ int pmx_engine_reset_stats(pmx_engine* e) {
 }
 
 } // extern "C"
+
+/* ------------------------------------------------------------- batches */
+/*
+ * Many reference views through the run<W> schedule at once
+ * (src/patchMatch.cu:307-407 per view).  Each of the 4 x iterations sweep
+ * steps is one persistent launch over the column strips of every view
+ * (pmx_sweep_batch_kernel); cost-all-views and the state transposes stay
+ * per view.  The whole schedule can be captured once in a hipGraph and
+ * replayed, which removes the per-launch host cost of the ~20 launches per
+ * view per iteration.
+ */
+struct pmx_batch {
+    pmx_ctx* ctx = nullptr;
+    std::vector<pmx_engine*> engs;
+    int waves = 1;            /* waves per column strip (1 or 2) */
+    int grid = 0;             /* persistent workgroups; 0 = every resident slot */
+    bool use_graph = true;
+    int steps = 0;            /* sweep steps of one run */
+    int max_V = 0;
+    /* device tables, steps x: n args, 8 x (n + 1) queue starts, 8 x n first columns */
+    PmxSweepArgs* d_tab = nullptr;
+    int* d_qstart = nullptr;
+    int* d_qcol = nullptr;
+    unsigned* d_queue = nullptr; /* steps x 8 */
+    std::vector<long long> sig; /* configuration the tables and graph were built for */
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void batch_drop_tables(pmx_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    dev_free(b->d_tab);
+    dev_free(b->d_qstart);
+    dev_free(b->d_qcol);
+    dev_free(b->d_queue);
+    b->d_tab = nullptr;
+    b->d_qstart = b->d_qcol = nullptr;
+    b->d_queue = nullptr;
+    b->sig.clear();
+}
+
+std::vector<long long> batch_signature(const pmx_batch* b) {
+    std::vector<long long> s{b->waves, b->grid, b->use_graph ? 1 : 0};
+    for (const pmx_engine* e : b->engs) {
+        s.push_back(e->p.num_iterations);
+        s.push_back(e->storage);
+        s.push_back(e->has_prop ? 1 : 0);
+    }
+    return s;
+}
+
+/* (down, rotated) of the four sweeps of an iteration (src/patchMatch.cu:344-391) */
+constexpr int kSweepDir[4][2] = {{1, 1}, {1, 0}, {0, 1}, {0, 0}};
+
+int batch_build_tables(pmx_batch* b) {
+    pmx_ctx* ctx = b->ctx;
+    const int n = (int)b->engs.size();
+    const int iters = b->engs[0]->p.num_iterations;
+    b->steps = 4 * iters;
+    std::vector<PmxSweepArgs> tab((size_t)b->steps * n);
+    std::vector<int> qstart((size_t)b->steps * 8 * (n + 1)), qcol((size_t)b->steps * 8 * n);
+    for (int k = 0; k < b->steps; k++) {
+        const int it = k / 4, down = kSweepDir[k % 4][0], rot = kSweepDir[k % 4][1];
+        const bool ids = it == iters - 1 && k % 4 == 3;
+        /* views in order of decreasing column length */
+        std::vector<int> order(n);
+        for (int v = 0; v < n; v++) order[v] = v;
+        auto len = [&](int v) { return rot ? b->engs[v]->W : b->engs[v]->H; };
+        std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return len(p) > len(q); });
+        int* qs = qstart.data() + (size_t)k * 8 * (n + 1);
+        int* qc = qcol.data() + (size_t)k * 8 * n;
+        for (int j = 0; j < n; j++) {
+            pmx_engine* e = b->engs[order[j]];
+            PmxSweepArgs a = sweep_args(e, rot);
+            a.down = down;
+            a.S = it == 0 ? 1 : e->p.num_samples; /* src/patchMatch.cu:328-331 */
+            a.ids = ids ? e->used_ids : nullptr;
+            a.sweep_id = k;
+            tab[(size_t)k * n + j] = a;
+        }
+        for (int x = 0; x < 8; x++) {
+            int acc = 0;
+            for (int j = 0; j < n; j++) {
+                const int Ws = tab[(size_t)k * n + j].Ws;
+                const int lo = (int)((long)Ws * x / 8), hi = (int)((long)Ws * (x + 1) / 8);
+                qs[x * (n + 1) + j] = acc;
+                qc[x * n + j] = lo;
+                acc += hi - lo;
+            }
+            qs[x * (n + 1) + n] = acc;
+        }
+    }
+    PMXCHK(dev_alloc(ctx, (void**)&b->d_tab, sizeof(PmxSweepArgs) * tab.size()));
+    PMXCHK(dev_alloc(ctx, (void**)&b->d_qstart, sizeof(int) * qstart.size()));
+    PMXCHK(dev_alloc(ctx, (void**)&b->d_qcol, sizeof(int) * qcol.size()));
+    PMXCHK(dev_alloc(ctx, (void**)&b->d_queue, sizeof(unsigned) * 8 * b->steps));
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(b->d_tab, tab.data(), sizeof(PmxSweepArgs) * tab.size(),
+                               hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(b->d_qstart, qstart.data(), sizeof(int) * qstart.size(),
+                               hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(b->d_qcol, qcol.data(), sizeof(int) * qcol.size(),
+                               hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return PMX_OK;
+}
+
+PmxBatchLaunch batch_launch(const pmx_batch* b, int k) {
+    const int n = (int)b->engs.size();
+    PmxBatchLaunch L{};
+    L.tab = b->d_tab + (size_t)k * n;
+    L.qstart = b->d_qstart + (size_t)k * 8 * (n + 1);
+    L.qcol = b->d_qcol + (size_t)k * 8 * n;
+    L.n = n;
+    L.queue = b->d_queue + 8 * k;
+    L.waves = b->waves;
+    L.hw = b->engs[0]->p.half_window;
+    L.max_V = b->max_V;
+    L.h16 = b->engs[0]->storage == PMX_STORE_F16;
+    return L;
+}
+
+/* Workgroups of the persistent launch: every resident slot unless set. */
+int batch_grid(pmx_batch* b, int* grid) {
+    if (b->grid > 0) {
+        *grid = b->grid;
+        return PMX_OK;
+    }
+    PmxBatchLaunch L = batch_launch(b, 0);
+    int per_cu = 0, cus = 0;
+    HIPCHK(b->ctx, pmx_launch_sweep_batch(nullptr, L, &per_cu));
+    HIPCHK(b->ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                         b->ctx->device));
+    if (per_cu < 1) return fail(b->ctx, PMX_ERR_HIP, "batched sweep kernel cannot be resident");
+    *grid = per_cu * cus;
+    return PMX_OK;
+}
+
+/* The run schedule of every view, enqueued on the context's stream (the
+ * caller may be capturing it).  Evaluation counts are kept by
+ * pmx_batch_run, which knows whether the work ran. */
+int batch_enqueue(pmx_batch* b, int grid) {
+    pmx_ctx* ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, pmx_launch_zero_u32(s, b->d_queue, 8 * b->steps));
+    for (int k = 0; k < b->steps; k++) {
+        const int rot = kSweepDir[k % 4][1];
+        for (pmx_engine* e : b->engs) {
+            if (k == 0) {
+                const unsigned long long ce = e->cost_all_evals, cs = e->cost_all_samples;
+                PMXCHK(do_cost_all(e, 1, nullptr));
+                e->cost_all_evals = ce;
+                e->cost_all_samples = cs;
+            }
+            PMXCHK(to_layout(e, rot ? 0 : 1, nullptr));
+        }
+        PmxBatchLaunch L = batch_launch(b, k);
+        L.grid = grid;
+        HIPCHK(ctx, pmx_launch_sweep_batch(s, L, nullptr));
+    }
+    for (pmx_engine* e : b->engs) PMXCHK(to_layout(e, 0, nullptr));
+    return PMX_OK;
+}
+
+int check_batch(pmx_batch* b) {
+    if (!b) return fail(nullptr, PMX_ERR_ARG, "batch is NULL");
+    return PMX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pmx_batch_create(pmx_ctx* ctx, pmx_engine* const* engines, int num_views, pmx_batch** out) {
+    if (!ctx) return fail(nullptr, PMX_ERR_ARG, "ctx is NULL");
+    if (!out || !engines) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_batch_create");
+    *out = nullptr;
+    if (num_views < 1 || num_views > 4096)
+        return fail(ctx, PMX_ERR_ARG, "num_views %d out of range 1..4096", num_views);
+    const pmx_engine* e0 = engines[0];
+    for (int v = 0; v < num_views; v++) {
+        const pmx_engine* e = engines[v];
+        if (!e) return fail(ctx, PMX_ERR_ARG, "engine %d is NULL", v);
+        if (e->ctx != ctx) return fail(ctx, PMX_ERR_ARG, "engine %d belongs to another context", v);
+        if (e->p.half_window != e0->p.half_window || e->p.num_samples != e0->p.num_samples ||
+            e->p.num_iterations != e0->p.num_iterations || e->storage != e0->storage)
+            return fail(ctx, PMX_ERR_ARG,
+                        "engine %d differs from engine 0 in half_window, num_samples, "
+                        "num_iterations or storage", v);
+        for (int u = 0; u < v; u++)
+            if (engines[u] == e) return fail(ctx, PMX_ERR_ARG, "engine %d appears twice", v);
+    }
+    pmx_batch* b = new pmx_batch();
+    b->ctx = ctx;
+    b->engs.assign(engines, engines + num_views);
+    for (const pmx_engine* e : b->engs) b->max_V = std::max(b->max_V, e->V);
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        if (b->ev0) (void)hipEventDestroy(b->ev0);
+        delete b;
+        return fail(ctx, PMX_ERR_HIP, "hipEventCreate failed");
+    }
+    *out = b;
+    return PMX_OK;
+}
+
+int pmx_batch_destroy(pmx_batch* b) {
+    if (!b) return PMX_OK;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    batch_drop_tables(b);
+    (void)hipEventDestroy(b->ev0);
+    (void)hipEventDestroy(b->ev1);
+    delete b;
+    return PMX_OK;
+}
+
+int pmx_batch_prepare(pmx_batch* b) {
+    PMXCHK(check_batch(b));
+    for (pmx_engine* e : b->engs) PMXCHK(prepare_impl(e, nullptr));
+    return PMX_OK;
+}
+
+int pmx_batch_run(pmx_batch* b, float* ms) {
+    PMXCHK(check_batch(b));
+    pmx_ctx* ctx = b->ctx;
+    for (pmx_engine* e : b->engs) {
+        if (e->storage != b->engs[0]->storage || e->p.num_iterations != b->engs[0]->p.num_iterations)
+            return fail(ctx, PMX_ERR_ARG, "engines of the batch no longer share storage and "
+                                          "num_iterations");
+        if (!e->prepared || e->sweep_count != 0 || e->layout_cm != 0)
+            return fail(ctx, PMX_ERR_STATE,
+                        "a batch runs freshly prepared engines: call pmx_batch_prepare first");
+    }
+    if (batch_signature(b) != b->sig) {
+        batch_drop_tables(b);
+        PMXCHK(batch_build_tables(b));
+        b->sig = batch_signature(b);
+    }
+    hipStream_t s = ctx->stream;
+    int grid = 0;
+    PMXCHK(batch_grid(b, &grid));
+    if (b->use_graph && !b->exec) {
+        hipGraph_t g = nullptr;
+        HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_enqueue(b, grid);
+        hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc != PMX_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIPCHK(ctx, ec);
+        hipError_t ei = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) b->exec = nullptr;
+        HIPCHK(ctx, ei);
+        /* capture recorded the layout changes without running them */
+        for (pmx_engine* e : b->engs) e->layout_cm = 0;
+    }
+    if (ms) HIPCHK(ctx, hipEventRecord(b->ev0, s));
+    if (b->use_graph) {
+        HIPCHK(ctx, hipGraphLaunch(b->exec, s));
+    } else {
+        PMXCHK(batch_enqueue(b, grid));
+    }
+    if (ms) {
+        HIPCHK(ctx, hipEventRecord(b->ev1, s));
+        HIPCHK(ctx, hipEventSynchronize(b->ev1));
+        HIPCHK(ctx, hipEventElapsedTime(ms, b->ev0, b->ev1));
+    }
+    for (pmx_engine* e : b->engs) {
+        e->layout_cm = 0;
+        e->sweep_count = b->steps;
+        e->stats_acc.sweep_launches += b->steps;
+        e->cost_all_evals += (unsigned long long)e->W * e->H * e->V;
+        e->cost_all_samples += window_samples(e->W, e->H, e->p.half_window) * (unsigned long long)e->V;
+    }
+    return PMX_OK;
+}
+
+int pmx_batch_set_graph(pmx_batch* b, int enabled) {
+    PMXCHK(check_batch(b));
+    b->use_graph = enabled != 0;
+    return PMX_OK;
+}
+
+int pmx_batch_set_launch(pmx_batch* b, int waves, int workgroups) {
+    PMXCHK(check_batch(b));
+    if (waves != 1 && waves != 2)
+        return fail(b->ctx, PMX_ERR_ARG, "waves per column strip must be 1 or 2 (got %d)", waves);
+    /* every one of the 8 XCD queues needs a workgroup b with b % 8 == x */
+    if (workgroups != 0 && workgroups < 8)
+        return fail(b->ctx, PMX_ERR_ARG, "workgroups must be 0 (auto) or at least 8 (got %d)",
+                    workgroups);
+    b->waves = waves;
+    b->grid = workgroups;
+    return PMX_OK;
+}
+
+int pmx_batch_get_launch(pmx_batch* b, int* waves, int* workgroups) {
+    PMXCHK(check_batch(b));
+    if (!waves || !workgroups) return fail(b->ctx, PMX_ERR_ARG, "NULL argument");
+    *waves = b->waves;
+    if (b->sig.empty() && !b->d_tab) {
+        /* the resident-slot count needs no tables */
+        int g = b->grid;
+        if (g == 0) {
+            PmxBatchLaunch L{};
+            L.waves = b->waves;
+            L.hw = b->engs[0]->p.half_window;
+            L.max_V = b->max_V;
+            L.h16 = b->engs[0]->storage == PMX_STORE_F16;
+            int per_cu = 0, cus = 0;
+            HIPCHK(b->ctx, pmx_launch_sweep_batch(nullptr, L, &per_cu));
+            HIPCHK(b->ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                                 b->ctx->device));
+            g = per_cu * cus;
+        }
+        *workgroups = g;
+        return PMX_OK;
+    }
+    return batch_grid(b, workgroups);
+}
+
+int pmx_batch_autotune(pmx_batch* b, int* waves, int* workgroups, float* ms) {
+    PMXCHK(check_batch(b));
+    const bool graph = b->use_graph;
+    b->use_graph = false;
+    float best = 0.0f;
+    int bw = 1, bg = 0;
+    int rc = PMX_OK;
+    for (int w = 1; w <= 2 && rc == PMX_OK; w++) {
+        b->waves = w;
+        b->grid = 0;
+        int full = 0;
+        rc = batch_grid(b, &full);
+        const int cands[3] = {full, (3 * full) / 4, full / 2};
+        for (int i = 0; i < 3 && rc == PMX_OK; i++) {
+            if (cands[i] < 8) continue;
+            b->grid = cands[i];
+            float t = 0.0f;
+            rc = pmx_batch_prepare(b);
+            if (rc == PMX_OK) rc = pmx_batch_run(b, &t);
+            if (rc == PMX_OK && (best == 0.0f || t < best)) {
+                best = t;
+                bw = w;
+                bg = cands[i];
+            }
+        }
+    }
+    b->use_graph = graph;
+    if (rc != PMX_OK) return rc;
+    b->waves = bw;
+    b->grid = bg;
+    if (waves) *waves = bw;
+    if (workgroups) *workgroups = bg;
+    if (ms) *ms = best;
+    return PMX_OK;
+}
+
+} // extern "C"

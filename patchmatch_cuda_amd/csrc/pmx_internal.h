@@ -75,6 +75,19 @@ struct PmxSweepArgs {
     double oX, oZ, iK0, iK1;
 };
 
+/* One batched sweep launch (pmx_sweep_batch_kernel): device tables of the
+ * views' arguments and of the per-XCD column queues. */
+struct PmxBatchLaunch {
+    const PmxSweepArgs* tab; /* n views */
+    const int* qstart;       /* 8 x (n + 1) */
+    const int* qcol;         /* 8 x n */
+    int n;
+    unsigned* queue;         /* 8 counters, zero before the launch */
+    int grid;                /* resident workgroups */
+    int waves;               /* 1 or 2 per column strip */
+    int hw, max_V, h16;
+};
+
 struct pmx_engine {
     pmx_ctx* ctx = nullptr;
     pmx_params p{};
@@ -165,6 +178,8 @@ hipError_t pmx_launch_cost_all(hipStream_t s, const PmxSweepArgs& a);
 hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a, int waves);
 hipError_t pmx_launch_cost_volume(hipStream_t s, const PmxSweepArgs& a, int view0, int view1,
                                   uint16_t* out, int W0, int H0);
+hipError_t pmx_launch_sweep_batch(hipStream_t s, const PmxBatchLaunch& b, int* per_cu);
+hipError_t pmx_launch_zero_u32(hipStream_t s, unsigned* p, int n);
 hipError_t pmx_launch_quantize_f16(hipStream_t s, float* data, uint16_t* out, size_t n);
 hipError_t pmx_launch_seed_depth(hipStream_t s, const float* prop, int prop_ld, float* depth,
                                  int depth_ld, int width, int height);

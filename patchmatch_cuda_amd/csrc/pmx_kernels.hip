@@ -1050,12 +1050,7 @@ __device__ __forceinline__ float proposal_draw(float depth, float spread, int x,
  * column's outputs.
  */
 template <int G, int PF, int WN, int NW, bool H16>
-/* A 2048-column sweep puts 8 workgroups on each of the 256 CUs: with NW
- * waves each that is 2*NW waves per SIMD, all of which must be resident at
- * once (a column that waits for a free slot doubles the sweep time), so
- * the register budget is capped at 512 / (2*NW) per lane. */
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2 * NW > 8 ? 8 : 2 * NW)))
-void pmx_sweep_kernel(PmxSweepArgs a) {
+__device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c) {
     extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
     const int WW = 2 * Wn * Wn;                          /* window + zero rows */
@@ -1072,8 +1067,6 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
     const int wave = (int)threadIdx.x / kLanes, lane = (int)threadIdx.x % kLanes;
     const bool lead = wave == 0;
     const int gl = lane % G;
-    const int c = xcd_remap(blockIdx.x, gridDim.x);
-    if (c >= a.Ws) return;
     for (int i = (int)threadIdx.x; i < 2 * WW; i += 64 * NW) winbuf[i] = 0.0f;
     load_homographies(a, Hs);
     const int V = a.V, Hs_rows = a.Hs;
@@ -1326,6 +1319,55 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
     }
 }
 
+/* A 2048-column sweep puts 8 workgroups on each of the 256 CUs: with NW
+ * waves each that is 2*NW waves per SIMD, all of which must be resident at
+ * once (a column that waits for a free slot doubles the sweep time), so
+ * the register budget is capped at 512 / (2*NW) per lane. */
+template <int G, int PF, int WN, int NW, bool H16>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2 * NW > 8 ? 8 : 2 * NW)))
+void pmx_sweep_kernel(PmxSweepArgs a) {
+    const int c = xcd_remap(blockIdx.x, gridDim.x);
+    if (c >= a.Ws) return;
+    sweep_column<G, PF, WN, NW, H16>(a, c);
+}
+
+/*
+ * One sweep step of many reference views in one persistent launch.  The
+ * grid is sized to the workgroups the device keeps resident; each
+ * workgroup takes column strips from the queue of its XCD until the queue
+ * is empty.  Queue x holds, for every view in the order of tab (longest
+ * columns first), the x-th of 8 contiguous ranges of that view's columns,
+ * so neighbouring columns (which read neighbouring source footprints) stay
+ * in one XCD's L2 and every XCD gets the same mix of column lengths.
+ *   tab[v]            the sweep arguments of view v
+ *   qstart[x*(n+1)+v] first queue slot of view v in queue x (qstart[.. n] = length)
+ *   qcol[x*n+v]       first column of view v's range x
+ *   queue[x]          next free slot of queue x (zero before the launch)
+ * Workgroup b runs on XCD b % 8 (round-robin dispatch), which only decides
+ * locality: every workgroup drains its queue, so every column is processed
+ * whatever the placement.
+ */
+template <int G, int PF, int WN, int NW, bool H16>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2 * NW > 8 ? 8 : 2 * NW)))
+void pmx_sweep_batch_kernel(const PmxSweepArgs* __restrict__ tab, const int* __restrict__ qstart,
+                            const int* __restrict__ qcol, int n, unsigned* __restrict__ queue) {
+    __shared__ int s_slot;
+    const int x = (int)(blockIdx.x & 7u);
+    const int* __restrict__ st = qstart + x * (n + 1);
+    const int total = st[n];
+    for (;;) {
+        if (threadIdx.x == 0) s_slot = (int)atomicAdd(queue + x, 1u);
+        __syncthreads();
+        const int j = s_slot;
+        __syncthreads();
+        if (j >= total) break;
+        int v = 0;
+        while (st[v + 1] <= j) v++;
+        sweep_column<G, PF, WN, NW, H16>(tab[v], qcol[x * n + v] + (j - st[v]));
+        __syncthreads();
+    }
+}
+
 inline dim3 grid2(int w, int h, dim3 b) { return dim3((w + b.x - 1) / b.x, (h + b.y - 1) / b.y); }
 
 } // namespace
@@ -1461,7 +1503,61 @@ hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
                            a);
     return hipGetLastError();
 }
+template <int G, int PF, int WN, int NW>
+hipError_t launch_sweep_batch_t(hipStream_t s, const PmxBatchLaunch& b) {
+    const int Wn = 2 * b.hw + 1;
+    const size_t sh = sizeof(double) * b.max_V * 18 + sizeof(float4) * 2 * 64 * NW +
+                      sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
+    if (b.h16)
+        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, true>), dim3(b.grid),
+                           dim3(64 * NW), sh, s, b.tab, b.qstart, b.qcol, b.n, b.queue);
+    else
+        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, false>), dim3(b.grid),
+                           dim3(64 * NW), sh, s, b.tab, b.qstart, b.qcol, b.n, b.queue);
+    return hipGetLastError();
+}
+
+template <int G, int PF, int WN, int NW>
+hipError_t occupancy_batch_t(const PmxBatchLaunch& b, int* per_cu) {
+    const int Wn = 2 * b.hw + 1;
+    const size_t sh = sizeof(double) * b.max_V * 18 + sizeof(float4) * 2 * 64 * NW +
+                      sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
+    if (b.h16)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, true>, 64 * NW, sh);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, false>, 64 * NW, sh);
+}
 } // namespace
+
+/* Batched sweeps: the same shapes as single-view sweeps with one or two
+ * waves per column strip.  per_cu != NULL: report the resident workgroups
+ * per CU instead of launching. */
+hipError_t pmx_launch_sweep_batch(hipStream_t s, const PmxBatchLaunch& b, int* per_cu) {
+    const int Wn = 2 * b.hw + 1;
+#define PMX_BATCH_NW(G, PF, WN)                                                              \
+    if (b.waves == 2)                                                                         \
+        return per_cu ? occupancy_batch_t<G, PF, WN, 2>(b, per_cu)                           \
+                      : launch_sweep_batch_t<G, PF, WN, 2>(s, b);                            \
+    return per_cu ? occupancy_batch_t<G, PF, WN, 1>(b, per_cu)                               \
+                  : launch_sweep_batch_t<G, PF, WN, 1>(s, b);
+    if (Wn == 15) { PMX_BATCH_NW(16, 4, 15) }
+    if (Wn == 11) { PMX_BATCH_NW(16, 2, 11) }
+    if (Wn <= 16) { PMX_BATCH_NW(16, 4, 0) }
+    if (Wn <= 32) { PMX_BATCH_NW(32, 16, 0) }
+    PMX_BATCH_NW(64, 44, 0)
+#undef PMX_BATCH_NW
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t pmx_launch_zero_u32(hipStream_t s, unsigned* p, int n) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
 
 /* Window sizes 11 and 15 (the configured workloads) get fully unrolled
  * row loops; every other size runs the same code with a dynamic trip
