@@ -98,7 +98,7 @@ def proposal_planes(ctx, eng, net, tau):
     x = capi.DeviceArray(ctx, (1, H0, W0, proposal.IN_C), np.uint16)
     d = capi.DeviceArray(ctx, (H, W), np.float32)
     sp = capi.DeviceArray(ctx, (H, W), np.float32)
-    eng.cost_volume(*eng.widest_views(), x.ptr)
+    eng.cost_volume(*eng.volume_views(), x.ptr)
     net.forward(x)
     net.propose(W, H, tau, d, sp)
     out = d.download(), sp.download()
@@ -185,30 +185,40 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau)
     ceng.close()
     pd.close()
     sd.close()
-    # full-view timing at k
-    eng.set_num_iterations(k)
-    for _ in range(warmup):
-        eng.prepare()
-        eng.propose(net, tau=tau)
-        eng.run()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        eng.prepare()
-        eng.propose(net, tau=tau)
-        eng.run()
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    full_gt = float(np.mean(np.abs(eng.depth() - sc.gt_depth) <= 0.01 * sc.gt_depth))
-    eng.set_proposal(None, 0)
-    eng.set_num_iterations(rp.num_iterations)
+    # full-view timing at k (learned) and at the plain method's own count
+    def timed(iters, learned):
+        eng.set_num_iterations(iters)
+        for i in range(warmup + steps):
+            if i == warmup:
+                ctx.sync()
+                t0 = time.perf_counter()
+            eng.prepare()
+            if learned:
+                eng.propose(net, tau=tau)
+            eng.run()
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        gt_share = float(np.mean(np.abs(eng.depth() - sc.gt_depth) <= 0.01 * sc.gt_depth))
+        eng.set_proposal(None, 0)
+        eng.set_num_iterations(rp.num_iterations)
+        return wall, gt_share
+
+    wall, full_gt = timed(k, True)
+    kp = need["plain"] or rp.num_iterations
+    wall_p, _ = timed(kp, False)
     net.close()
-    log(f"learned: {k} iteration(s), {wall * 1e3 / steps:.2f} ms/step")
+    log(f"learned: {k} iteration(s), {wall * 1e3 / steps:.2f} ms/step; plain at {kp} "
+        f"iteration(s): {wall_p * 1e3 / steps:.2f} ms/step")
     return {
         "iterations": k,
         "sweeps": 4 * k,
         "ms_per_step": wall * 1e3 / steps,
         "value": eng.W * eng.H * steps / wall / 1e6,
+        "proposal_coverage": float(np.mean(prop > 0)),
+        "plain_at_its_99pct_count": {"iterations": kp,
+                                     "ms_per_step": wall_p * 1e3 / steps,
+                                     "value": eng.W * eng.H * steps / wall_p / 1e6},
+        "speedup_vs_plain_at_equal_accuracy": wall_p / wall,
         "within_1pct_of_ground_truth": full_gt,
         "iterations_to_99pct_of_converged_oracle": need,
         "sweep_study": study,
@@ -412,11 +422,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads (default: min(16, cpu count))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sweep-waves", type=int, default=0,
-                    help="wavefronts per sweep column (1, 2, 4; 0 = engine default)")
+    ap.add_argument("--sweep-waves", type=int, default=None,
+                    help="wavefronts per sweep column (1, 2, 4; 0 = automatic; default: the "
+                         "engine's)")
     ap.add_argument("--weights", default=None,
                     help="proposal network weights (default: the packaged file)")
-    ap.add_argument("--tau", type=float, default=0.3,
+    ap.add_argument("--tau", type=float, default=0.0,
                     help="confidence threshold of the learned proposal")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--fp16", action="store_true", help="FP16 texel storage")
@@ -442,7 +453,7 @@ def main():
         f"{time.perf_counter() - t0:.1f} s")
     ctx = capi.Context(0)
     eng = capi.Engine(ctx, rp, sc.ref, list(sc.srcs), sc.cams)
-    if args.sweep_waves:
+    if args.sweep_waves is not None:
         eng.set_sweep_waves(args.sweep_waves)
     if args.fp16:
         eng.set_storage(capi.STORE_F16)

@@ -172,6 +172,14 @@ int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format,
                       const int* src_pitch_bytes, const pmx_camera* ref_cam,
                       const pmx_camera* src_cams, pmx_engine** out);
 int pmx_engine_destroy(pmx_engine* eng);
+/* Device bytes an engine of these sizes holds (pmx_engine_create, plus the
+ * FP16 block of PMX_STORE_F16 and the planes of a proposal), without
+ * touching a device: the admission measure of a caller that keeps many
+ * views resident within an HBM budget.  No reference counterpart (the
+ * reference sizes its buffers inline, src/patchMatch.cu:51-206). */
+int pmx_engine_footprint(const pmx_params* params, int ref_width, int ref_height, int num_src,
+                         const int* src_widths, const int* src_heights, int storage,
+                         int with_proposal, size_t* bytes);
 /* Constructor-time device work, re-runnable from the resident images:
  * pre-blur, reference box sums, RNG init, initial depth (src/patchMatch.cu:
  * 160-179). */
@@ -207,9 +215,11 @@ int pmx_engine_export(pmx_engine* eng, float** depth, int* depth_pitch,
                       int* ids_pitch);
 int pmx_engine_set_timing(pmx_engine* eng, int enabled);
 /* Launch shape of the sweep kernel: wavefronts per sweep column (1, 2 or
- * 4).  No reference counterpart (its sweeps use one 32-thread block per 32
- * columns, src/patchMatch.cu:334).  Results do not depend on it beyond
- * floating-point summation order. */
+ * 4), or 0 = automatic (two waves per column where every column still fits
+ * three waves per SIMD, e.g. the 1536-column sweeps of a 2048x1536 view;
+ * one elsewhere).  No reference counterpart (its sweeps use one 32-thread
+ * block per 32 columns, src/patchMatch.cu:334).  Results do not depend on
+ * it beyond floating-point summation order. */
 int pmx_engine_set_sweep_waves(pmx_engine* eng, int waves);
 int pmx_engine_get_sweep_waves(pmx_engine* eng, int* waves);
 int pmx_engine_get_stats(pmx_engine* eng, pmx_stats* out);
@@ -283,9 +293,10 @@ int pmx_engine_cost_volume(pmx_engine* eng, int view0, int view1, void* x_dev);
  * initial depth. */
 int pmx_engine_set_proposal(pmx_engine* eng, const float* depth_dev, const float* spread_dev,
                             int pitch_bytes);
-/* The two source views farthest from the reference camera (widest
- * baselines): the cost volume's default pair. */
-int pmx_engine_widest_views(pmx_engine* eng, int* view0, int* view1);
+/* The two source views nearest to the reference camera (shortest
+ * baselines): the cost volume's default pair, the one the packaged network
+ * was trained on. */
+int pmx_engine_volume_views(pmx_engine* eng, int* view0, int* view1);
 /* Whether a proposal also seeds the depth (default 1). */
 int pmx_engine_set_proposal_seeding(pmx_engine* eng, int seed_depth);
 /* numOfIterations after construction (the proposal needs fewer). */
@@ -332,7 +343,7 @@ int pmx_net_propose(pmx_net* net, int width, int height, float tau, float* depth
                     int depth_pitch_bytes, float* spread_dev, int spread_pitch_bytes,
                     float* conf_dev, int conf_pitch_bytes);
 /* Cost volume -> forward -> proposal -> pmx_engine_set_proposal.  view0 or
- * view1 < 0 selects pmx_engine_widest_views. */
+ * view1 < 0 selects pmx_engine_volume_views. */
 int pmx_engine_propose(pmx_engine* eng, pmx_net* net, int view0, int view1, float tau);
 
 /* Single operators of the network, on device bf16 NHWC tensors (channel

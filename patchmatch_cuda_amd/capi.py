@@ -124,11 +124,13 @@ SIGNATURES = {
     # learned proposal
     "pmx_engine_cost_volume": [_vp, _i, _i, _vp],
     "pmx_engine_set_proposal": [_vp, _vp, _vp, _i],
-    "pmx_engine_widest_views": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "pmx_engine_volume_views": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "pmx_engine_set_num_iterations": [_vp, _i],
     "pmx_engine_set_proposal_seeding": [_vp, _i],
     "pmx_engine_propose": [_vp, _vp, _i, _i, _f],
     "pmx_engine_set_storage": [_vp, _i],
+    "pmx_engine_footprint": [ctypes.POINTER(Params), _i, _i, _i, _vp, _vp, _i, _i,
+                             ctypes.POINTER(ctypes.c_size_t)],
     "pmx_batch_create": [_vp, _vp, _i, ctypes.POINTER(_vp)],
     "pmx_batch_destroy": [_vp],
     "pmx_batch_prepare": [_vp],
@@ -430,9 +432,9 @@ class Engine:
                                               _vp(spread_ptr) if spread_ptr else None,
                                               pitch_bytes))
 
-    def widest_views(self):
+    def volume_views(self):
         a, b = ctypes.c_int(), ctypes.c_int()
-        self._c(lib().pmx_engine_widest_views(self.h, ctypes.byref(a), ctypes.byref(b)))
+        self._c(lib().pmx_engine_volume_views(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
     def set_proposal_seeding(self, on=True):
@@ -456,6 +458,18 @@ class Engine:
         v = ctypes.c_int()
         self._c(lib().pmx_engine_get_storage(self.h, ctypes.byref(v)))
         return v.value
+
+
+def engine_footprint(params, width, height, src_sizes, storage=STORE_F32, proposal=False) -> int:
+    """Device bytes of an engine (pmx_engine_footprint); src_sizes: [(w, h)]."""
+    ws = np.ascontiguousarray([w for w, _ in src_sizes], np.int32)
+    hs = np.ascontiguousarray([h for _, h in src_sizes], np.int32)
+    p = Params.from_run_params(params)
+    out = ctypes.c_size_t()
+    check(lib().pmx_engine_footprint(ctypes.byref(p), int(width), int(height), len(src_sizes),
+                                     _vp(ws.ctypes.data), _vp(hs.ctypes.data), int(storage),
+                                     int(bool(proposal)), ctypes.byref(out)))
+    return out.value
 
 
 class Batch:

@@ -1322,9 +1322,11 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
 /* A 2048-column sweep puts 8 workgroups on each of the 256 CUs: with NW
  * waves each that is 2*NW waves per SIMD, all of which must be resident at
  * once (a column that waits for a free slot doubles the sweep time), so
- * the register budget is capped at 512 / (2*NW) per lane. */
-template <int G, int PF, int WN, int NW, bool H16>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2 * NW > 8 ? 8 : 2 * NW)))
+ * the register budget is capped at 512 / WPE per lane, WPE = 2*NW.  A
+ * 1536-column sweep with two waves per column needs 3 waves per SIMD
+ * (WPE = 3), which leaves 168 registers instead of 128. */
+template <int G, int PF, int WN, int NW, bool H16, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE)))
 void pmx_sweep_kernel(PmxSweepArgs a) {
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= a.Ws) return;
@@ -1491,16 +1493,17 @@ hipError_t launch_volume_t(hipStream_t s, const PmxSweepArgs& a, int v0, int v1,
     return hipGetLastError();
 }
 
-template <int G, int PF, int WN, int NW>
+template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW)>
 hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
     const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * 2 * 64 * NW +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     if (a.src_h16)
-        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, true>), dim3(a.Ws), dim3(64 * NW), sh, s, a);
+        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, true, WPE>), dim3(a.Ws), dim3(64 * NW),
+                           sh, s, a);
     else
-        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, false>), dim3(a.Ws), dim3(64 * NW), sh, s,
-                           a);
+        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, false, WPE>), dim3(a.Ws), dim3(64 * NW),
+                           sh, s, a);
     return hipGetLastError();
 }
 template <int G, int PF, int WN, int NW>
@@ -1604,6 +1607,11 @@ hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a, int waves) {
     case 4: return launch_sweep_t<G, PF, WN, 4>(s, a);               \
     default: return launch_sweep_t<G, PF, WN, 2>(s, a);              \
     }
+    /* waves == 3: two waves per column at three waves per SIMD (the
+     * configured window sizes only) */
+    if (waves == 3 && Wn == 15) return launch_sweep_t<16, 4, 15, 2, 3>(s, a);
+    if (waves == 3 && Wn == 11) return launch_sweep_t<16, 2, 11, 2, 3>(s, a);
+    if (waves == 3) waves = 1;
     if (Wn == 15) { PMX_SWEEP_NW(16, 4, 15) }
     if (Wn == 11) { PMX_SWEEP_NW(16, 2, 11) }
     if (Wn <= 16) { PMX_SWEEP_NW(16, 4, 0) }

@@ -1,0 +1,212 @@
+"""Serving loop: a seeded Poisson stream of MVS requests run on one GPU.
+
+Each request is one reference view with 4-20 source views at one of five
+resolutions from 640x480 to 3072x2048, FP16 texels, and (in the learned
+mode) the learned proposal.  The loop admits waiting requests in arrival
+order while the device bytes of the resident views (pmx_engine_footprint)
+stay within an HBM budget, keeps each admitted view's images and state
+resident on the device from upload to result, and runs everything admitted
+as one batch (pmx_batch_run: one persistent sweep launch per sweep step
+over all views).  New arrivals wait for the running batch.  Latency is
+arrival -> depth map on the host.
+
+The reference serves nothing (one reference view per process,
+src/main.cpp:96-127); this is the request path around its runPatchMatch
+(src/patchMatch.cu:263-407).
+
+Scenes are rendered before the loop starts: one 20-source scene per
+resolution, of which a request with V sources uses the first V (the
+rendering cost would otherwise be part of the measured latency).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from collections import deque
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi, configs, proposal, scene
+
+SIZES = [(640, 480), (1024, 768), (1600, 1200), (2048, 1536), (3072, 2048)]
+MAX_SRC = 20
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float       # seconds from the start of the loop
+    size: int            # index into SIZES
+    n_src: int
+
+
+def make_stream(rate, duration, seed=11, min_src=4, max_src=MAX_SRC):
+    """Poisson arrivals at `rate` per second over `duration` seconds."""
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration:
+            return out
+        out.append(Request(len(out), t, int(rng.integers(len(SIZES))),
+                           int(rng.integers(min_src, max_src + 1))))
+
+
+def scene_pool(seed=21):
+    """One rendered 20-source scene per resolution (C2's world statistics)."""
+    pool = []
+    for i, (W, H) in enumerate(SIZES):
+        spec = configs.SceneSpec(W, H, MAX_SRC, 20, 5, 1.0, 20.0, 1800.0 * W / 2048.0, 0.1, 1.0,
+                                 15.0, 0.01, seed + i, seed + 100 + i)
+        pool.append(scene.render(spec))
+    return pool
+
+
+def _pct(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
+
+
+class Server:
+    def __init__(self, ctx, rp, pool, budget_bytes, learned_net=None, learned_iterations=None,
+                 tau=0.3, max_batch=32):
+        self.ctx, self.rp, self.pool = ctx, rp, pool
+        self.budget = int(budget_bytes)
+        self.net = learned_net
+        self.iters = learned_iterations or rp.num_iterations
+        self.tau = tau
+        self.max_batch = max_batch
+        self.nets = {}
+
+    def footprint(self, req):
+        W, H = SIZES[req.size]
+        return capi.engine_footprint(self.rp, W, H, [(W, H)] * req.n_src, capi.STORE_F16,
+                                     proposal=self.net is not None)
+
+    def _net_for(self, W, H):
+        """The proposal network sized for one resolution (weights shared by
+        loading the same file)."""
+        if (W, H) not in self.nets:
+            W0, H0 = proposal.low_res(W, H)
+            n = proposal.Net(self.ctx, 1, H0, W0, self.rp.depth_near, self.rp.depth_far)
+            n.set_params(self.net.params())
+            self.nets[(W, H)] = n
+        return self.nets[(W, H)]
+
+    def _admit(self, req):
+        sc = self.pool[req.size]
+        eng = capi.Engine(self.ctx, self.rp, sc.ref, list(sc.srcs[:req.n_src]),
+                          sc.cams[:req.n_src + 1])
+        eng.set_storage(capi.STORE_F16)
+        if self.net is not None:
+            eng.set_num_iterations(self.iters)
+        return eng
+
+    def run(self, stream, log=print):
+        pending = deque(stream)
+        waiting = deque()
+        lat, done_px, batches, sizes = [], 0, 0, []
+        resident = 0
+        t0 = time.perf_counter()
+        last_log = 0.0
+        n_total = len(stream)
+        while pending or waiting:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            if not waiting:
+                time.sleep(max(0.0, min(pending[0].arrival - now, 0.05)))
+                continue
+            # admission: arrival order, within the HBM budget
+            admitted = []
+            while waiting and len(admitted) < self.max_batch:
+                fp = self.footprint(waiting[0])
+                if resident + fp > self.budget and admitted:
+                    break
+                req = waiting.popleft()
+                resident += fp
+                admitted.append((req, fp, self._admit(req)))
+            engs = [e for _, _, e in admitted]
+            b = capi.Batch(self.ctx, engs)
+            b.set_graph(False)
+            b.prepare()
+            if self.net is not None:
+                for req, _, e in admitted:
+                    W, H = SIZES[req.size]
+                    e.propose(self._net_for(W, H), tau=self.tau)
+            b.run()
+            for req, fp, e in admitted:
+                e.depth()                    # the result reaches the host
+                lat.append(time.perf_counter() - t0 - req.arrival)
+                W, H = SIZES[req.size]
+                done_px += W * H
+                e.close()
+                resident -= fp
+            b.close()
+            batches += 1
+            sizes.append(len(admitted))
+            now = time.perf_counter() - t0
+            if now - last_log > 10.0:
+                log(f"t={now:.0f}s: {len(lat)}/{n_total} done, {len(waiting)} waiting, "
+                    f"p50 latency {_pct(lat, 50):.2f}s")
+                last_log = now
+        wall = time.perf_counter() - t0
+        for n in self.nets.values():
+            n.close()
+        self.nets.clear()
+        return {
+            "requests": len(lat),
+            "wall_s": wall,
+            "throughput_req_s": len(lat) / wall,
+            "throughput_mpix_s": done_px / wall / 1e6,
+            "latency_s": {"p50": _pct(lat, 50), "p95": _pct(lat, 95), "p99": _pct(lat, 99),
+                          "max": float(max(lat)) if lat else float("nan")},
+            "batches": batches,
+            "mean_batch": float(np.mean(sizes)) if sizes else 0.0,
+        }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="Poisson-stream serving loop (learned vs plain)")
+    ap.add_argument("--rate", type=float, default=4.0, help="requests per second")
+    ap.add_argument("--duration", type=float, default=600.0, help="seconds of arrivals")
+    ap.add_argument("--budget-gib", type=float, default=64.0, help="HBM budget for views")
+    ap.add_argument("--learned-iterations", type=int, default=2)
+    ap.add_argument("--tau", type=float, default=0.3)
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--weights", default=proposal.DEFAULT_WEIGHTS)
+    ap.add_argument("--modes", default="learned,plain")
+    a = ap.parse_args(argv)
+    log = lambda *m: print(*m, file=sys.stderr, flush=True)  # noqa: E731
+    _, rp = configs.C2_REALISTIC
+    t = time.perf_counter()
+    pool = scene_pool()
+    log(f"scene pool rendered in {time.perf_counter() - t:.1f} s")
+    stream = make_stream(a.rate, a.duration, a.seed)
+    ctx = capi.Context(0)
+    out = {"rate_req_s": a.rate, "duration_s": a.duration, "budget_gib": a.budget_gib,
+           "requests": len(stream), "seed": a.seed}
+    for mode in a.modes.split(","):
+        net = None
+        if mode == "learned":
+            W0, H0 = proposal.low_res(*SIZES[0])
+            net = proposal.Net(ctx, 1, H0, W0, rp.depth_near, rp.depth_far)
+            net.load(a.weights)
+        srv = Server(ctx, rp, pool, a.budget_gib * 2 ** 30, net, a.learned_iterations, a.tau)
+        log(f"mode {mode}: {len(stream)} requests over {a.duration:.0f} s")
+        res = srv.run(stream, log)
+        res["iterations"] = srv.iters
+        out[mode] = res
+        log(f"mode {mode}: {json.dumps(res)}")
+        if net is not None:
+            net.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -203,3 +203,36 @@ def test_full_run_c1_depth_agreement(c1):
     _say(f"  within 1% of oracle: {agree:.4f}; of ground truth: gpu {gt:.4f} "
          f"oracle {within(do, sc.gt_depth, 0.01):.4f}")
     assert agree >= 0.99
+
+
+def test_full_run_c1_auto_waves(c1):
+    """The automatic launch shape (two waves per column at three waves per
+    SIMD wherever the columns fit) meets the same bar, and its teacher-forced
+    sweep agrees decision for decision."""
+    spec, rp, sc, eng = c1
+    _say("case: full run c1, automatic sweep waves")
+    eng.set_sweep_waves(0)
+    try:
+        eng.prepare()
+        eng.cost_all_views(rotated=True)
+        eng.sweep(1, 1, 1)
+        depth, costs, states = eng.depth(), eng.costs(), eng.rng()
+        pb = oracle_for(eng, rp, sc)
+        d_s = depth.astype(np.float64)
+        ids_o, tr = pb.sweep(d_s, costs.astype(np.float64), states.copy(), 1, 0, rp.num_samples,
+                             record_ids=True, trace=True)
+        eng.sweep(1, 0, rp.num_samples, record_ids=True)
+        res = compare_sweep(eng.depth(), d_s, eng.used_ids(), ids_o, tr, 1)
+        _say(f"  sweep: compared {res['compared']}/{res['total']} mismatches {res['mismatches']}")
+        assert res["mismatches"] == 0
+        assert res["compared"] >= 0.6 * res["total"]
+        eng.prepare()
+        d0, st0 = eng.depth(), eng.rng()
+        eng.run()
+        dg = eng.depth()
+        do, _, _ = pb.run(d0.astype(np.float64), st0.copy(), rp.num_samples, rp.num_iterations)
+        agree = within(dg, do, 0.01)
+        _say(f"  within 1% of oracle: {agree:.4f}")
+        assert agree >= 0.99
+    finally:
+        eng.set_sweep_waves(1)
