@@ -117,7 +117,7 @@ def crop_problem(sc, sample):
     return (x0, y0, w, h), scene.crop_reference(sc, x0, y0, w, h)
 
 
-def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau):
+def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau, seeding):
     """Learned-proposal PatchMatch against plain PatchMatch.
 
     Sweeps needed: on the centred crop, each method runs 1..iterations
@@ -141,6 +141,8 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau)
     prop = np.ascontiguousarray(prop[y0:y0 + h, x0:x0 + w])
     spread = np.ascontiguousarray(spread[y0:y0 + h, x0:x0 + w])
     ceng = capi.Engine(ctx, rp, ref, list(srcs), cams)
+    ceng.set_proposal_seeding(seeding)
+    eng.set_proposal_seeding(seeding)
     pd = capi.DeviceArray(ctx, (h, w), np.float32).upload(prop)
     sd = capi.DeviceArray(ctx, (h, w), np.float32).upload(spread)
     agree = lambda d: float(np.mean(np.abs(d - d_oracle) <= 0.01 * np.abs(d_oracle)))  # noqa
@@ -226,6 +228,7 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau)
                    "sample": f"{w}x{h} centred crop, {k} iteration(s), oracle given the same "
                              f"proposal planes"},
         "tau": tau,
+        "proposal_seeds_depth": bool(seeding),
         "weights": os.path.relpath(weights, ROOT),
     }
 
@@ -266,13 +269,21 @@ def batch_bench(args):
             dt = time.perf_counter() - t
         return npx / dt / 1e6, dt * 1e3
 
-    seq32, seq32_ms = one_at_a_time(capi.STORE_F32)
-    log(f"one view at a time, FP32: {seq32:.2f} Mpix/s")
-    seq16, seq16_ms = one_at_a_time(capi.STORE_F16)
-    log(f"one view at a time, FP16: {seq16:.2f} Mpix/s")
+    seq32 = seq32_ms = seq16 = seq16_ms = None
+    if not args.no_cpu_baseline:
+        seq32, seq32_ms = one_at_a_time(capi.STORE_F32)
+        log(f"one view at a time, FP32: {seq32:.2f} Mpix/s")
+        seq16, seq16_ms = one_at_a_time(capi.STORE_F16)
+        log(f"one view at a time, FP16: {seq16:.2f} Mpix/s")
+    else:
+        for e in engs:
+            e.set_storage(capi.STORE_F16)
     b = capi.Batch(ctx, engs)
-    waves, wgs, tune_ms = b.autotune()
-    log(f"autotuned: {waves} wave(s) per column strip, {wgs} workgroups ({tune_ms:.1f} ms)")
+    if args.no_autotune:
+        waves, wgs = b.launch()
+    else:
+        waves, wgs, tune_ms = b.autotune()
+        log(f"autotuned: {waves} wave(s) per column strip, {wgs} workgroups ({tune_ms:.1f} ms)")
     b.set_graph(True)
     for _ in range(args.warmup):
         b.prepare()
@@ -290,6 +301,30 @@ def batch_bench(args):
     b.close()
     for e in engs:
         e.close()
+    line = {
+        "metric": "batched depth Mpix/s (16 views, 10 src views each, mixed resolution)",
+        "value": value,
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 texels, f32 sums, f64 homographies",
+        "data": "synthetic (16 seeded scenes, no dataset)",
+        "config": {"workload": "c3_batch", "views": [f"{e[0].W}x{e[0].H}" for e in views],
+                   "src_views": 10, "window": 15, "iterations": 4, "num_samples": 5,
+                   "graph": True, "waves_per_strip": waves, "workgroups": wgs},
+        "gpu_ms_per_run": run_ms / args.steps,
+    }
+    if args.no_cpu_baseline:
+        ctx.close()
+        return line
+    line["one_view_at_a_time"] = {"fp32_mpix_s": seq32, "fp32_ms": seq32_ms,
+                                  "fp16_mpix_s": seq16, "fp16_ms": seq16_ms}
+    line["speedup_vs_one_at_a_time_fp32"] = value / seq32
 
     # per-view error on centred crops
     cw, ch = parse_sample(args.batch_crop)
@@ -334,29 +369,9 @@ def batch_bench(args):
     for e in cengs:
         e.close()
     ctx.close()
-    return {
-        "metric": "batched depth Mpix/s (16 views, 10 src views each, mixed resolution)",
-        "value": value,
-        "unit": "Mpix/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f16 texels, f32 sums, f64 homographies",
-        "data": "synthetic (16 seeded scenes, no dataset)",
-        "config": {"workload": "c3_batch", "views": [f"{e[0].W}x{e[0].H}" for e in views],
-                   "src_views": 10, "window": 15, "iterations": 4, "num_samples": 5,
-                   "graph": True, "waves_per_strip": waves, "workgroups": wgs},
-        "gpu_ms_per_run": run_ms / args.steps,
-        "one_view_at_a_time": {"fp32_mpix_s": seq32, "fp32_ms": seq32_ms,
-                               "fp16_mpix_s": seq16, "fp16_ms": seq16_ms},
-        "speedup_vs_one_at_a_time_fp32": value / seq32,
-        "per_view_error": {"sample": f"centred {cw}x{ch} crop of each view, batched, FP16",
-                           "views": per_view},
-    }
+    line["per_view_error"] = {"sample": f"centred {cw}x{ch} crop of each view, batched, FP16",
+                              "views": per_view}
+    return line
 
 
 def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads):
@@ -427,10 +442,14 @@ def main():
                          "engine's)")
     ap.add_argument("--weights", default=None,
                     help="proposal network weights (default: the packaged file)")
-    ap.add_argument("--tau", type=float, default=0.0,
+    ap.add_argument("--tau", type=float, default=0.2,
                     help="confidence threshold of the learned proposal")
+    ap.add_argument("--proposal-seeding", type=int, default=0,
+                    help="1: the learned proposal also seeds the initial depth")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--fp16", action="store_true", help="FP16 texel storage")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched workload: keep the default launch shape")
     ap.add_argument("--batch-crop", default="128x96",
                     help="per-view crop of the batched workload's oracle check")
     args = ap.parse_args()
@@ -530,7 +549,7 @@ def main():
         line["learned"] = {"skipped": "needs the oracle crop (--no-cpu-baseline given)"}
     else:
         lr = learned_leg(ctx, eng, sc, rp, weights, d_oracle, parse_sample(args.cpu_sample),
-                         args.steps, args.warmup, args.tau)
+                         args.steps, args.warmup, args.tau, bool(args.proposal_seeding))
         lr["speedup_vs_plain"] = lr["value"] / value
         lr["sweep_reduction_vs_plain"] = (4 * rp.num_iterations) / lr["sweeps"]
         line["learned"] = lr

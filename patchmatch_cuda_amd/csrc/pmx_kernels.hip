@@ -494,20 +494,22 @@ __device__ __forceinline__ TapW tap_fetch(const SB& sb, const Hyp& q, float sx, 
     return TapW{__builtin_amdgcn_fractf(sx), __builtin_amdgcn_fractf(sy)};
 }
 
-/* Two FP16 texels (low half first) widened to FP32, exactly. */
-__device__ __forceinline__ float2 tex_f32(uint32_t w) {
-    const _Float16 lo = __builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
-    const _Float16 hi = __builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
-    return make_float2((float)lo, (float)hi);
+/* Horizontal blend of a texel pair, (1 - al) lo + al hi, in FP32.  FP16
+ * pairs (low half first) are blended as lo + al hi - al lo: two FMAs whose
+ * FP16 operands the hardware widens itself (v_fma_mix_f32 selects either
+ * half of the loaded word), so no conversion instructions are needed. */
+__device__ __forceinline__ float row_blend(float al, float2 t) { return fmaf(al, t.y - t.x, t.x); }
+__device__ __forceinline__ float row_blend(float al, uint32_t w) {
+    const float lo = (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+    const float hi = (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+    return fmaf(-al, lo, fmaf(al, hi, lo));
 }
-__device__ __forceinline__ float2 tex_f32(float2 t) { return t; }
 
 /* ... and their bilinear blend, in FP32. */
 template <class T>
 __device__ __forceinline__ float tap_blend(const TapW& w, T r0, T r1) {
-    const float2 t0 = tex_f32(r0), t1 = tex_f32(r1);
-    const float top = fmaf(w.al, t0.y - t0.x, t0.x);
-    const float bot = fmaf(w.al, t1.y - t1.x, t1.x);
+    const float top = row_blend(w.al, r0);
+    const float bot = row_blend(w.al, r1);
     return fmaf(w.be, bot - top, top);
 }
 

@@ -7,8 +7,9 @@ order while the device bytes of the resident views (pmx_engine_footprint)
 stay within an HBM budget, keeps each admitted view's images and state
 resident on the device from upload to result, and runs everything admitted
 as one batch (pmx_batch_run: one persistent sweep launch per sweep step
-over all views).  New arrivals wait for the running batch.  Latency is
-arrival -> depth map on the host.
+over all views).  While a batch runs, the next group of arrivals is
+uploaded, prepared and proposed on a second context (its own HIP stream).
+Latency is arrival -> depth map on the host.
 
 The reference serves nothing (one reference view per process,
 src/main.cpp:96-127); this is the request path around its runPatchMatch
@@ -71,83 +72,115 @@ def _pct(a, q):
 
 
 class Server:
-    def __init__(self, ctx, rp, pool, budget_bytes, learned_net=None, learned_iterations=None,
-                 tau=0.3, max_batch=32):
-        self.ctx, self.rp, self.pool = ctx, rp, pool
+    """Runs a request stream.  Two contexts (two HIP streams) alternate: while
+    the batch admitted last runs on one, the next group of waiting requests
+    is uploaded, prepared and proposed on the other, so host-side admission
+    work overlaps the sweeps."""
+
+    def __init__(self, ctxs, rp, pool, budget_bytes, learned_weights=None,
+                 learned_iterations=None, tau=0.2, seeding=False, max_batch=32):
+        self.ctxs, self.rp, self.pool = list(ctxs), rp, pool
         self.budget = int(budget_bytes)
-        self.net = learned_net
+        self.weights = learned_weights
         self.iters = learned_iterations or rp.num_iterations
         self.tau = tau
+        self.seeding = seeding
         self.max_batch = max_batch
         self.nets = {}
+        self.resident = 0
 
     def footprint(self, req):
         W, H = SIZES[req.size]
         return capi.engine_footprint(self.rp, W, H, [(W, H)] * req.n_src, capi.STORE_F16,
-                                     proposal=self.net is not None)
+                                     proposal=self.weights is not None)
 
-    def _net_for(self, W, H):
-        """The proposal network sized for one resolution (weights shared by
-        loading the same file)."""
-        if (W, H) not in self.nets:
+    def _net_for(self, ctx, W, H):
+        """The proposal network for one context and resolution."""
+        key = (id(ctx), W, H)
+        if key not in self.nets:
             W0, H0 = proposal.low_res(W, H)
-            n = proposal.Net(self.ctx, 1, H0, W0, self.rp.depth_near, self.rp.depth_far)
-            n.set_params(self.net.params())
-            self.nets[(W, H)] = n
-        return self.nets[(W, H)]
+            n = proposal.Net(ctx, 1, H0, W0, self.rp.depth_near, self.rp.depth_far)
+            n.load(self.weights)
+            self.nets[key] = n
+        return self.nets[key]
 
-    def _admit(self, req):
-        sc = self.pool[req.size]
-        eng = capi.Engine(self.ctx, self.rp, sc.ref, list(sc.srcs[:req.n_src]),
-                          sc.cams[:req.n_src + 1])
-        eng.set_storage(capi.STORE_F16)
-        if self.net is not None:
-            eng.set_num_iterations(self.iters)
-        return eng
+    def _stage(self, ctx, waiting):
+        """Admits waiting requests in arrival order within the HBM budget
+        (at least one), uploads them on ctx, prepares them and installs the
+        learned proposal.  Returns the staged group or None."""
+        if not waiting:
+            return None
+        admitted = []
+        while waiting and len(admitted) < self.max_batch:
+            fp = self.footprint(waiting[0])
+            if self.resident + fp > self.budget and (admitted or self.resident > 0):
+                break
+            req = waiting.popleft()
+            self.resident += fp
+            sc = self.pool[req.size]
+            eng = capi.Engine(ctx, self.rp, sc.ref, list(sc.srcs[:req.n_src]),
+                              sc.cams[:req.n_src + 1])
+            eng.set_storage(capi.STORE_F16)
+            if self.weights is not None:
+                eng.set_num_iterations(self.iters)
+                eng.set_proposal_seeding(self.seeding)
+            admitted.append((req, fp, eng))
+        if not admitted:
+            return None
+        b = capi.Batch(ctx, [e for _, _, e in admitted])
+        b.set_graph(False)
+        b.prepare()
+        if self.weights is not None:
+            for req, _, e in admitted:
+                e.propose(self._net_for(ctx, *SIZES[req.size]), tau=self.tau)
+        return {"batch": b, "admitted": admitted}
+
+    def _finish(self, st, t0, lat):
+        """Waits for a launched group, reads every depth map back, frees it."""
+        px = 0
+        for req, fp, e in st["admitted"]:
+            e.depth()                    # synchronises the group's stream
+            lat.append(time.perf_counter() - t0 - req.arrival)
+            W, H = SIZES[req.size]
+            px += W * H
+        st["batch"].close()
+        for req, fp, e in st["admitted"]:
+            e.close()
+            self.resident -= fp
+        return px
 
     def run(self, stream, log=print):
         pending = deque(stream)
         waiting = deque()
-        lat, done_px, batches, sizes = [], 0, 0, []
-        resident = 0
+        lat, done_px, sizes = [], 0, []
         t0 = time.perf_counter()
         last_log = 0.0
         n_total = len(stream)
-        while pending or waiting:
+        turn = 0
+        staged = None
+
+        def arrive():
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            if not waiting:
-                time.sleep(max(0.0, min(pending[0].arrival - now, 0.05)))
-                continue
-            # admission: arrival order, within the HBM budget
-            admitted = []
-            while waiting and len(admitted) < self.max_batch:
-                fp = self.footprint(waiting[0])
-                if resident + fp > self.budget and admitted:
-                    break
-                req = waiting.popleft()
-                resident += fp
-                admitted.append((req, fp, self._admit(req)))
-            engs = [e for _, _, e in admitted]
-            b = capi.Batch(self.ctx, engs)
-            b.set_graph(False)
-            b.prepare()
-            if self.net is not None:
-                for req, _, e in admitted:
-                    W, H = SIZES[req.size]
-                    e.propose(self._net_for(W, H), tau=self.tau)
-            b.run()
-            for req, fp, e in admitted:
-                e.depth()                    # the result reaches the host
-                lat.append(time.perf_counter() - t0 - req.arrival)
-                W, H = SIZES[req.size]
-                done_px += W * H
-                e.close()
-                resident -= fp
-            b.close()
-            batches += 1
-            sizes.append(len(admitted))
+            return now
+
+        while pending or waiting or staged:
+            now = arrive()
+            if staged is None:
+                if not waiting:
+                    time.sleep(max(0.0, min(pending[0].arrival - now, 0.05)))
+                    continue
+                staged = self._stage(self.ctxs[turn % len(self.ctxs)], waiting)
+                turn += 1
+            running = staged
+            running["batch"].run()           # enqueued; returns at once
+            arrive()
+            staged = self._stage(self.ctxs[turn % len(self.ctxs)], waiting)
+            if staged is not None:
+                turn += 1
+            done_px += self._finish(running, t0, lat)
+            sizes.append(len(running["admitted"]))
             now = time.perf_counter() - t0
             if now - last_log > 10.0:
                 log(f"t={now:.0f}s: {len(lat)}/{n_total} done, {len(waiting)} waiting, "
@@ -164,7 +197,7 @@ class Server:
             "throughput_mpix_s": done_px / wall / 1e6,
             "latency_s": {"p50": _pct(lat, 50), "p95": _pct(lat, 95), "p99": _pct(lat, 99),
                           "max": float(max(lat)) if lat else float("nan")},
-            "batches": batches,
+            "batches": len(sizes),
             "mean_batch": float(np.mean(sizes)) if sizes else 0.0,
         }
 
@@ -174,8 +207,8 @@ def main(argv=None):
     ap.add_argument("--rate", type=float, default=4.0, help="requests per second")
     ap.add_argument("--duration", type=float, default=600.0, help="seconds of arrivals")
     ap.add_argument("--budget-gib", type=float, default=64.0, help="HBM budget for views")
-    ap.add_argument("--learned-iterations", type=int, default=2)
-    ap.add_argument("--tau", type=float, default=0.3)
+    ap.add_argument("--learned-iterations", type=int, default=4)
+    ap.add_argument("--tau", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--weights", default=proposal.DEFAULT_WEIGHTS)
     ap.add_argument("--modes", default="learned,plain")
@@ -186,24 +219,20 @@ def main(argv=None):
     pool = scene_pool()
     log(f"scene pool rendered in {time.perf_counter() - t:.1f} s")
     stream = make_stream(a.rate, a.duration, a.seed)
-    ctx = capi.Context(0)
+    ctxs = [capi.Context(0), capi.Context(0)]
     out = {"rate_req_s": a.rate, "duration_s": a.duration, "budget_gib": a.budget_gib,
-           "requests": len(stream), "seed": a.seed}
+           "requests": len(stream), "seed": a.seed,
+           "sizes": [f"{w}x{h}" for w, h in SIZES], "sources": "4-20"}
     for mode in a.modes.split(","):
-        net = None
-        if mode == "learned":
-            W0, H0 = proposal.low_res(*SIZES[0])
-            net = proposal.Net(ctx, 1, H0, W0, rp.depth_near, rp.depth_far)
-            net.load(a.weights)
-        srv = Server(ctx, rp, pool, a.budget_gib * 2 ** 30, net, a.learned_iterations, a.tau)
+        srv = Server(ctxs, rp, pool, a.budget_gib * 2 ** 30,
+                     a.weights if mode == "learned" else None, a.learned_iterations, a.tau)
         log(f"mode {mode}: {len(stream)} requests over {a.duration:.0f} s")
         res = srv.run(stream, log)
         res["iterations"] = srv.iters
         out[mode] = res
         log(f"mode {mode}: {json.dumps(res)}")
-        if net is not None:
-            net.close()
-    ctx.close()
+    for c in ctxs:
+        c.close()
     print(json.dumps(out), flush=True)
     return 0
 
