@@ -534,6 +534,21 @@ def main():
     }
     line["plain"] = {"value": value, "ms_per_step": ms_step, "sweeps": 4 * rp.num_iterations,
                      "within_1pct_of_ground_truth": plain_gt}
+    # host buffers in, depth map out: upload of the reference and sources,
+    # device allocation, the step, readback (never the headline value)
+    ctx.sync()
+    t0 = time.perf_counter()
+    e2 = capi.Engine(ctx, rp, sc.ref, list(sc.srcs), sc.cams)
+    e2.prepare()
+    e2.run()
+    e2.depth()
+    t1 = time.perf_counter()
+    e2.close()
+    line["pcie_inclusive"] = {"value": npx / (t1 - t0) / 1e6, "unit": "Mpix/s",
+                              "ms": (t1 - t0) * 1e3,
+                              "what": "engine creation (allocation + upload of 11 float32 "
+                                      "images) + one step + depth readback"}
+    log(f"PCIe-inclusive: {(t1 - t0) * 1e3:.1f} ms")
     weights = args.weights or proposal.DEFAULT_WEIGHTS
     d_oracle = None
     if not args.no_cpu_baseline:

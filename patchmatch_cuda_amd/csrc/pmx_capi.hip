@@ -1508,9 +1508,15 @@ int pmx_batch_run(pmx_batch* b, float* ms) {
         HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         const int rc = batch_enqueue(b, grid);
         hipError_t ec = hipStreamEndCapture(s, &g);
-        if (rc != PMX_OK) {
+        if (rc != PMX_OK || ec != hipSuccess) {
+            /* nothing ran: undo the layout bookkeeping of the capture and
+             * require a new prepare */
             if (g) (void)hipGraphDestroy(g);
-            return rc;
+            for (pmx_engine* e : b->engs) {
+                e->layout_cm = 0;
+                e->prepared = false;
+            }
+            if (rc != PMX_OK) return rc;
         }
         HIPCHK(ctx, ec);
         hipError_t ei = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
@@ -1524,7 +1530,13 @@ int pmx_batch_run(pmx_batch* b, float* ms) {
     if (b->use_graph) {
         HIPCHK(ctx, hipGraphLaunch(b->exec, s));
     } else {
-        PMXCHK(batch_enqueue(b, grid));
+        const int rc = batch_enqueue(b, grid);
+        if (rc != PMX_OK) {
+            /* part of the schedule may be enqueued: the state is undefined
+             * until the next prepare */
+            for (pmx_engine* e : b->engs) e->prepared = false;
+            return rc;
+        }
     }
     if (ms) {
         HIPCHK(ctx, hipEventRecord(b->ev1, s));

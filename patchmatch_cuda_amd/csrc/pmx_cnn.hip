@@ -343,7 +343,7 @@ __global__ void head_loss_kernel(const float* __restrict__ Hf, const float* __re
                                  uint16_t* __restrict__ dH, int P, int C, float lo, float hi,
                                  float inv_count, float w_conf, float* __restrict__ sums) {
     const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    float l1 = 0.0f, bce = 0.0f;
+    float l1 = 0.0f, lb = 0.0f;
     if (p < P) {
         const float z0 = Hf[(size_t)p * C], z1 = Hf[(size_t)p * C + 1];
         const float s0 = 1.0f / (1.0f + __expf(-z0));
@@ -357,7 +357,7 @@ __global__ void head_loss_kernel(const float* __restrict__ Hf, const float* __re
             l1 = fabsf(e);
             g0 = (e > 0.0f ? 1.0f : (e < 0.0f ? -1.0f : 0.0f)) * (hi - lo) * s0 * (1.0f - s0) * inv_count;
             const float eb = b - fabsf(e);
-            bce = fabsf(eb);
+            lb = fabsf(eb);
             g1 = w_conf * (eb > 0.0f ? 1.0f : (eb < 0.0f ? -1.0f : 0.0f)) * (kScaleHi - kScaleLo) *
                  s1 * (1.0f - s1) * inv_count;
         }
@@ -367,11 +367,11 @@ __global__ void head_loss_kernel(const float* __restrict__ Hf, const float* __re
     /* block sums */
     for (int o = 32; o >= 1; o >>= 1) {
         l1 += __shfl_xor(l1, o);
-        bce += __shfl_xor(bce, o);
+        lb += __shfl_xor(lb, o);
     }
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&sums[0], l1);
-        atomicAdd(&sums[1], bce);
+        atomicAdd(&sums[1], lb);
     }
 }
 

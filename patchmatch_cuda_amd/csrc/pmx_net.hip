@@ -1,8 +1,8 @@
 /*
  * pmx_net.hip -- the proposal network: a 4-level BF16 encoder-decoder of 3x3
  * convolutions (channels 32/64/128/256) on the MFMA kernels of pmx_cnn.hip,
- * with its training step (L1 on inverse depth + confidence BCE, Adam) and the
- * C ABI declared in include/pmx.h.
+ * with its training step (L1 on inverse depth + L1 of the predicted error
+ * scale, Adam) and the C ABI declared in include/pmx.h.
  *
  * Input (1/4 resolution, NHWC, 64 channels): 32 plane-sweep costs + the
  * reference intensity + 31 zero channels.  Output (head, FP32, 32 channels):
