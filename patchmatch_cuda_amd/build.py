@@ -58,9 +58,25 @@ def _atomic(out, cmd_builder):
 
 
 def build_pmx(extra=()):
-    srcs = [os.path.join(CSRC, f) for f in ("pmx_kernels.hip", "pmx_capi.hip", "pmx_cnn.hip",
-                                            "pmx_net.hip")]
-    _atomic(LIB_PMX, lambda o: [HIPCC, *HIP_FLAGS, *extra, "-shared", "-o", o, *srcs])
+    """Compiles the HIP sources to objects in parallel, then links libpmx.so."""
+    names = ("pmx_kernels.hip", "pmx_capi.hip", "pmx_cnn.hip", "pmx_net.hip")
+    objdir = os.path.join(ROOT, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = []
+    for n in names:
+        obj = os.path.join(objdir, n.replace(".hip", ".o"))
+        cmd = [HIPCC, *HIP_FLAGS, *extra, "-c", "-o", obj, os.path.join(CSRC, n)]
+        print("+", " ".join(cmd), flush=True)
+        jobs.append((cmd, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                stderr=subprocess.STDOUT, text=True)))
+    objs = []
+    for cmd, obj, pr in jobs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            raise RuntimeError(f"build step failed ({pr.returncode}): {' '.join(cmd[:3])} ...")
+        objs.append(obj)
+    _atomic(LIB_PMX, lambda o: [HIPCC, *HIP_FLAGS, "-shared", "-o", o, *objs])
 
 
 def build_scene():

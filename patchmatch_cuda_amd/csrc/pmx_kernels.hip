@@ -1301,9 +1301,16 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         }
         n_samples += (unsigned long long)npx * (unsigned long long)(nt + (idx != 1 ? V - U : 0));
 
-        /* forward message with the updated costs (src/patchMatch.cu:996-999) */
-        e = sentinel_emission(cnew, is_view, lane, rng);
-        m = msg_forward(m_prev, e, mc);
+        /* forward message with the updated costs (src/patchMatch.cu:996-999).
+         * When the current depth won, the costs are unchanged: without a
+         * sentinel (which draws again) the message is the one computed at
+         * the top of the row, bit for bit. */
+        if (idx == 1 && !__ballot(is_view && cst == kSentinel)) {
+            m = mf;
+        } else {
+            e = sentinel_emission(cnew, is_view, lane, rng);
+            m = msg_forward(m_prev, e, mc);
+        }
         d_former = best;
 
         /* next window into the other buffer */
