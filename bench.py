@@ -539,6 +539,8 @@ def main():
     ctx.sync()
     t0 = time.perf_counter()
     e2 = capi.Engine(ctx, rp, sc.ref, list(sc.srcs), sc.cams)
+    if args.fp16:
+        e2.set_storage(capi.STORE_F16)
     e2.prepare()
     e2.run()
     e2.depth()

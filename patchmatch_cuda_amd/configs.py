@@ -71,20 +71,21 @@ C2_REALISTIC = (
 
 # Precision + batching: 16 reference views, each with 10 sources, resolutions
 # drawn uniformly from four sizes, FP16 texel storage, one batched run.
-C3_SIZES = [(1024, 768), (1600, 1200), (2048, 1536), (3072, 2048)]
+BATCH_SIZES = [(1024, 768), (1600, 1200), (2048, 1536), (3072, 2048)]
 
 
 def c3_batch(n_views=16, seed=3):
     """The 16 views of the batched workload: (SceneSpec, RunParams) pairs.
-    Each view is its own seeded world with C2's statistics (20 planes + 5
-    spheres, depths 1-20, baselines 0.1-1.0, rotations within 15 deg), the
-    focal length scaled with the width (1800 at 2048)."""
+    Each view is its own seeded world with the statistics of the 2048x1536
+    workload (20 planes + 5 spheres, depths 1-20, baselines 0.1-1.0,
+    rotations within 15 deg), the focal length scaled with the width (1800
+    at 2048)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     _, rp = C2_REALISTIC
     out = []
     for i in range(n_views):
-        W, H = C3_SIZES[int(rng.integers(len(C3_SIZES)))]
+        W, H = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
         out.append((SceneSpec(W, H, 10, 20, 5, 1.0, 20.0, 1800.0 * W / 2048.0, 0.1, 1.0, 15.0,
                               0.01, 300 + i, 400 + i), rp))
     return out

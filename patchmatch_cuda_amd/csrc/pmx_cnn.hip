@@ -262,16 +262,27 @@ __global__ __launch_bounds__(256) void conv_bwd_weight_kernel(const uint16_t* __
 }
 
 /* db[c] += sum_p dZ[p][c] */
+/* db[c] += sum over pixels of dZ[., c]: each thread sums a strided set of
+ * pixels of one channel, the block reduces its rows in LDS, and one atomic
+ * per channel per block adds the block's sum (one atomic per thread made
+ * 256 K atomics on at most 256 addresses contend). */
 __global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t* __restrict__ dZ, float* db,
                                                         int P, int C) {
+    __shared__ float part[256];
     const int c = threadIdx.x % C;
     const int rows = 256 / C;
     const int r0 = threadIdx.x / C;
-    if (r0 >= rows) return;
     float s = 0.0f;
-    for (int p = (int)blockIdx.x * rows + r0; p < P; p += (int)gridDim.x * rows)
-        s += bf2f(dZ[(size_t)p * C + c]);
-    atomicAdd(&db[c], s);
+    if (r0 < rows)
+        for (int p = (int)blockIdx.x * rows + r0; p < P; p += (int)gridDim.x * rows)
+            s += bf2f(dZ[(size_t)p * C + c]);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if ((int)threadIdx.x < C) {
+        float t = 0.0f;
+        for (int r = 0; r < rows; r++) t += part[r * C + threadIdx.x];
+        atomicAdd(&db[threadIdx.x], t);
+    }
 }
 
 /* dY *= (Y > 0) */
@@ -505,7 +516,8 @@ hipError_t pmx_launch_conv_bwd_weight(hipStream_t s, const uint16_t* X, const ui
 }
 
 hipError_t pmx_launch_bias_grad(hipStream_t s, const uint16_t* dZ, float* db, int P, int C) {
-    const unsigned blocks = std::min<unsigned>(nblk(P, 256 / C), 1024u);
+    if (C < 1 || C > 256) return hipErrorInvalidValue; /* one channel per lane of a block */
+    const unsigned blocks = std::min<unsigned>(nblk(P, 256 / C), 512u);
     bias_grad_kernel<<<blocks, 256, 0, s>>>(dZ, db, P, C);
     return hipGetLastError();
 }

@@ -174,8 +174,9 @@ class Net:
 
 # ------------------------------------------------------------ training data
 def training_spec(i, W=640, H=480, V=6, seed0=10_000):
-    """Scene i of the training set: C2's world statistics at 640x480 (focal
-    scaled with the width), 6 sources, its own seeds."""
+    """Scene i of the training set: the 2048x1536 workload's world statistics
+    at 640x480 (focal length scaled with the width), 6 sources, its own
+    seeds."""
     return configs.SceneSpec(W, H, V, 20, 5, 1.0, 20.0, 1800.0 * W / 2048.0, 0.1, 1.0, 15.0,
                              0.01, seed0 + i, seed0 + 50_000 + i)
 

@@ -58,7 +58,8 @@ def make_stream(rate, duration, seed=11, min_src=4, max_src=MAX_SRC):
 
 
 def scene_pool(seed=21):
-    """One rendered 20-source scene per resolution (C2's world statistics)."""
+    """One rendered 20-source scene per resolution, with the world statistics
+    of the 2048x1536 workload."""
     pool = []
     for i, (W, H) in enumerate(SIZES):
         spec = configs.SceneSpec(W, H, MAX_SRC, 20, 5, 1.0, 20.0, 1800.0 * W / 2048.0, 0.1, 1.0,
@@ -88,9 +89,15 @@ class Server:
         self.max_batch = max_batch
         self.nets = {}
         self.resident = 0
+        self.keep_results = False
+        self.results = {}
+
+    def dims(self, req):
+        spec = self.pool[req.size].spec
+        return spec.W, spec.H
 
     def footprint(self, req):
-        W, H = SIZES[req.size]
+        W, H = self.dims(req)
         return capi.engine_footprint(self.rp, W, H, [(W, H)] * req.n_src, capi.STORE_F16,
                                      proposal=self.weights is not None)
 
@@ -132,16 +139,18 @@ class Server:
         b.prepare()
         if self.weights is not None:
             for req, _, e in admitted:
-                e.propose(self._net_for(ctx, *SIZES[req.size]), tau=self.tau)
+                e.propose(self._net_for(ctx, *self.dims(req)), tau=self.tau)
         return {"batch": b, "admitted": admitted}
 
     def _finish(self, st, t0, lat):
         """Waits for a launched group, reads every depth map back, frees it."""
         px = 0
         for req, fp, e in st["admitted"]:
-            e.depth()                    # synchronises the group's stream
+            d = e.depth()                # synchronises the group's stream
+            if self.keep_results:
+                self.results[req.rid] = d
             lat.append(time.perf_counter() - t0 - req.arrival)
-            W, H = SIZES[req.size]
+            W, H = self.dims(req)
             px += W * H
         st["batch"].close()
         for req, fp, e in st["admitted"]:

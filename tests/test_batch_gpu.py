@@ -113,8 +113,8 @@ def test_batch_fp16(gpu_ctx):
 
 
 def test_batch_member_against_oracle(gpu_ctx):
-    """C1 (320x240, 4 sources) batched with a tiny view: >= 99% of its final
-    depths within 1% of the FP64 oracle on the same inputs."""
+    """The 320x240, 4-source workload batched with a tiny view: >= 99% of its
+    final depths within 1% of the FP64 oracle on the same inputs."""
     _say("case: batch member (c1) against the oracle")
     spec, rp = configs.C1_SMALL
     sc, eng = make_engine(gpu_ctx, spec, rp, prepare=False)
@@ -152,7 +152,9 @@ def test_batch_errors(gpu_ctx):
     with pytest.raises(capi.PmxError):
         b.run()                                     # not freshly prepared
     with pytest.raises(capi.PmxError):
-        b.set_launch(3, 0)
+        b.set_launch(3, 0)                          # 1 or 2 waves per strip
+    with pytest.raises(capi.PmxError):
+        b.set_launch(1, 4)                          # fewer workgroups than XCD queues
     b.close()
     a[3].close()
     other.close()

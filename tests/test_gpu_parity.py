@@ -1,6 +1,6 @@
 """GPU parity of every hot-path stage against the FP64 oracle.
 
-Bars (north star): transposes, RNG draws and selected-view indices
+Bars: transposes, RNG draws and selected-view indices
 bit-exact; NCC costs within 1e-5 absolute; >= 99% of final depths within 1%
 relative.  Stage tests feed the oracle the engine's own upstream values
 (teacher forcing) and compare decisions up to the first row whose oracle
