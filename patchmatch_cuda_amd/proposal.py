@@ -173,11 +173,13 @@ class Net:
 
 
 # ------------------------------------------------------------ training data
-def training_spec(i, W=640, H=480, V=6, seed0=10_000):
+def training_spec(i, W=640, H=480, V=6, seed0=10_000, focal=None):
     """Scene i of the training set: the 2048x1536 workload's world statistics
-    at 640x480 (focal length scaled with the width), 6 sources, its own
-    seeds."""
-    return configs.SceneSpec(W, H, V, 20, 5, 1.0, 20.0, 1800.0 * W / 2048.0, 0.1, 1.0, 15.0,
+    at 640x480, 6 sources, its own seeds.  focal None scales the focal length
+    with the width (the same field of view); a fixed focal length keeps the
+    2048x1536 view's pixel disparities in a narrower field of view."""
+    f = 1800.0 * W / 2048.0 if focal is None else float(focal)
+    return configs.SceneSpec(W, H, V, 20, 5, 1.0, 20.0, f, 0.1, 1.0, 15.0,
                              0.01, seed0 + i, seed0 + 50_000 + i)
 
 
@@ -187,7 +189,7 @@ def training_params(V=6):
 
 
 def make_dataset(ctx, n, W=640, H=480, V=6, seed0=10_000, threads=16, log=print,
-                 views=None):
+                 views=None, focal=None):
     """Renders n scenes, runs the plain engine on each and returns device
     arrays x (n, H0, W0, 64) bf16 and tgt (n, H0, W0) float32."""
     from . import scene as scene_mod
@@ -198,7 +200,8 @@ def make_dataset(ctx, n, W=640, H=480, V=6, seed0=10_000, threads=16, log=print,
     rp = training_params(V)
     t0 = time.time()
     with ThreadPoolExecutor(max_workers=threads) as pool:
-        futs = [pool.submit(scene_mod.render, training_spec(i, W, H, V, seed0)) for i in range(n)]
+        futs = [pool.submit(scene_mod.render, training_spec(i, W, H, V, seed0, focal))
+                for i in range(n)]
         for i, fut in enumerate(futs):
             sc = fut.result()
             futs[i] = None
