@@ -54,3 +54,38 @@ def test_served_depths_equal_standalone_runs(gpu_ctx):
         assert np.array_equal(srv.results[r.rid], want), r
         assert np.array_equal(srv2.results[r.rid], want), r
     ctx2.close()
+
+
+def test_footprint_matches_device_allocation(gpu_ctx):
+    """pmx_engine_footprint (the admission measure) against the device memory
+    an engine actually takes: hipMemGetInfo before and after creating an
+    FP16 engine with a proposal installed."""
+    import ctypes
+
+    _say("case: engine footprint vs hipMemGetInfo")
+    # the HIP runtime instance libpmx.so already loaded
+    path = next(ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64" in ln)
+    hip = ctypes.CDLL(path)
+    free_b, total_b = ctypes.c_size_t(), ctypes.c_size_t()
+
+    def free_now():
+        gpu_ctx.sync()
+        assert hip.hipMemGetInfo(ctypes.byref(free_b), ctypes.byref(total_b)) == 0
+        return free_b.value
+
+    spec, rp = configs.tiny(W=1024, H=768, V=6, hw=7, S=3, iters=1, seed=4)
+    sc = scene.render(spec)
+    fp = capi.engine_footprint(rp, spec.W, spec.H, [(spec.W, spec.H)] * spec.V,
+                               capi.STORE_F16, proposal=True)
+    f0 = free_now()
+    eng = capi.Engine(gpu_ctx, rp, sc.ref, list(sc.srcs), sc.cams)
+    eng.set_storage(capi.STORE_F16)
+    prop = capi.DeviceArray(gpu_ctx, (spec.H, spec.W), np.float32)
+    eng.set_proposal(prop.ptr, spec.W * 4)
+    f2 = free_now()
+    used = (f0 - f2) - prop.nbytes
+    _say(f"  footprint {fp / 2**20:.1f} MiB, measured {used / 2**20:.1f} MiB")
+    # the device allocator rounds each of the ~25 allocations up to its page
+    assert 0.97 * fp <= used <= fp + 64 * 2**20
+    eng.close()
+    prop.close()
