@@ -374,8 +374,10 @@ def batch_bench(args):
     return line
 
 
-def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads):
-    """Oracle (FP64, OpenMP) and the GPU engine on the same reference crop."""
+def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads, fp16=False):
+    """Oracle (FP64, OpenMP) and the GPU engine on the same reference crop.
+    With fp16 the engine stores FP16 texels while the oracle reads the FP32
+    images: the agreement is then the error FP16 storage adds."""
     from oracle import pmx_oracle as O
     from patchmatch_cuda_amd import capi, scene
 
@@ -411,6 +413,8 @@ def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads):
     }
     if do_accuracy:
         eng = capi.Engine(ctx, rp, ref, list(srcs), cams)
+        if fp16:
+            eng.set_storage(capi.STORE_F16)
         eng.prepare()
         eng.run()
         d_g = eng.depth().astype(np.float64)
@@ -418,7 +422,8 @@ def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads):
         agree = float(np.mean(np.abs(d_g - d_o) <= 0.01 * np.abs(d_o)))
         out["accuracy"] = {
             "within_1pct_of_fp64_oracle": agree,
-            "sample": f"{w}x{h} centred crop, same engine and parameters",
+            "sample": f"{w}x{h} centred crop, same engine and parameters"
+                      + (", FP16 texels against the oracle on FP32 images" if fp16 else ""),
             "gpu_within_1pct_of_ground_truth": float(np.mean(np.abs(d_g - gt) <= 0.01 * gt)),
             "oracle_within_1pct_of_ground_truth": float(np.mean(np.abs(d_o - gt) <= 0.01 * gt)),
         }
@@ -556,7 +561,7 @@ def main():
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         extra, d_oracle = cpu_baseline_and_accuracy(sc, rp, parse_sample(args.cpu_sample), ctx,
-                                                    True, threads)
+                                                    True, threads, args.fp16)
         line.update(extra)
     if args.no_learned:
         pass
