@@ -92,7 +92,7 @@ def test_fp16_cost_all_views_within_1e5(c1_16):
          f"p99.9 {np.quantile(diff, 0.999):.3g}")
     assert np.mean((cg == 2.0) == (co == 2.0)) == 1.0
     assert np.quantile(diff, 0.999) < 1e-5
-    assert diff.max() < 1e-3
+    assert diff.max() < 1e-5
     # precision: the oracle on the FP32 (unrounded) images
     ref32 = O.blur(sc.ref, rp.gaussian_sigma, rp.blur_mode)
     src32 = np.stack([O.blur(s, rp.gaussian_sigma, rp.blur_mode) for s in sc.srcs])

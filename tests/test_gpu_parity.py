@@ -101,7 +101,7 @@ def test_cost_all_views_within_1e5(c1):
              f"depth={depth[y, x]:.5f}")
     assert sentinel_agree == 1.0
     assert np.quantile(diff, 0.999) < 1e-5
-    assert diff.max() < 1e-3
+    assert diff.max() < 1e-5
 
 
 @pytest.mark.parametrize("rotated", [0, 1])
