@@ -201,7 +201,8 @@ int pmx_engine_get_rng(pmx_engine* eng, pmx_rng_state* host);
 int pmx_engine_set_rng(pmx_engine* eng, const pmx_rng_state* host);
 int pmx_engine_get_used_ids(pmx_engine* eng, uint8_t* host);
 /* Pre-blurred images the NCC reads: reference W x H, sources V planes of
- * maxW x maxH (zero outside each source). */
+ * maxW x maxH (zero outside each source).  Like the window sums below, they
+ * exist from pmx_engine_prepare on (PMX_ERR_STATE before). */
 int pmx_engine_get_images(pmx_engine* eng, float* ref_host, float* src_host);
 int pmx_engine_get_ref_sums(pmx_engine* eng, float* sum_I, float* sum_II);
 int pmx_engine_get_homographies(pmx_engine* eng, double* host);

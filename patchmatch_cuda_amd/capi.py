@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -207,9 +208,19 @@ class Context:
         h = _vp()
         check(lib().pmx_ctx_create(device, ctypes.byref(h)))
         self.h = h
+        self._deps = weakref.WeakSet()
+
+    def track(self, obj):
+        """Registers an object that holds resources of this context (engine,
+        batch, network, device array): close() releases it first, so nothing
+        outlives the context it points into."""
+        self._deps.add(obj)
 
     def close(self):
         if getattr(self, "h", None):
+            # batches borrow engines; arrays and engines go before the context
+            for dep in sorted(list(getattr(self, "_deps", ())), key=lambda d: d.CLOSE_ORDER):
+                dep.close()
             lib().pmx_ctx_destroy(self.h)
             self.h = None
 
@@ -232,6 +243,7 @@ class DeviceArray:
     synchronous uploads/downloads and stream-ordered device copies on the
     context's stream.  dtype is the host view: float32, uint16 (bf16 bits),
     uint8."""
+    CLOSE_ORDER = 2
 
     def __init__(self, ctx: "Context", shape, dtype=np.float32, zero=True):
         self.ctx = ctx
@@ -242,6 +254,7 @@ class DeviceArray:
         h = _vp()
         ctx.check(lib().pmx_malloc(ctx.h, ctypes.byref(h), max(self.nbytes, 1)))
         self.h = h
+        ctx.track(self)
         if zero:
             ctx.check(lib().pmx_memset(ctx.h, self.h, 0, self.nbytes))
 
@@ -301,6 +314,7 @@ def _p(a):
 
 class Engine:
     """One reference view against V source views, resident on the GPU."""
+    CLOSE_ORDER = 1
 
     def __init__(self, ctx: Context, params, ref, srcs, cams, image_format=IMG_F32):
         self.ctx = ctx
@@ -330,6 +344,7 @@ class Engine:
         ctx.check(lib().pmx_engine_get_dims(self.h, *[ctypes.byref(x) for x in d]))
         self.maxW, self.maxH = d[3].value, d[4].value
         self.S = self.params.num_samples
+        ctx.track(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -475,6 +490,7 @@ def engine_footprint(params, width, height, src_sizes, storage=STORE_F32, propos
 class Batch:
     """Several engines run together (pmx_batch_*): one persistent sweep
     launch per sweep step over every view, replayed from a hipGraph."""
+    CLOSE_ORDER = 0
 
     def __init__(self, ctx: Context, engines):
         self.ctx = ctx
@@ -483,6 +499,7 @@ class Batch:
         h = _vp()
         ctx.check(lib().pmx_batch_create(ctx.h, arr, len(self.engines), ctypes.byref(h)))
         self.h = h
+        ctx.track(self)
 
     def close(self):
         if getattr(self, "h", None):

@@ -1150,6 +1150,7 @@ int pmx_engine_get_used_ids(pmx_engine* e, uint8_t* host) {
 
 int pmx_engine_get_images(pmx_engine* e, float* ref_host, float* src_host) {
     PMXCHK(check_engine(e));
+    PMXCHK(require_prepared(e)); /* the pre-blurred images exist from prepare on */
     hipStream_t s = e->ctx->stream;
     if (ref_host)
         HIPCHK(e->ctx, hipMemcpy2DAsync(ref_host, (size_t)e->W * 4, e->ref, (size_t)e->ref_pitch * 4,
@@ -1168,6 +1169,7 @@ int pmx_engine_get_images(pmx_engine* e, float* ref_host, float* src_host) {
 
 int pmx_engine_get_ref_sums(pmx_engine* e, float* sum_I, float* sum_II) {
     PMXCHK(check_engine(e));
+    PMXCHK(require_prepared(e));
     hipStream_t s = e->ctx->stream;
     if (sum_I)
         HIPCHK(e->ctx, hipMemcpy2DAsync(sum_I, (size_t)e->W * 4, e->sum_I, (size_t)e->ref_pitch * 4,

@@ -59,6 +59,7 @@ def low_res(W, H):
 
 class Net:
     """The proposal network on one context (pmx_net_*)."""
+    CLOSE_ORDER = 1
 
     def __init__(self, ctx: capi.Context, max_batch: int, H0: int, W0: int, near: float,
                  far: float, seed: int = 0):
@@ -68,6 +69,7 @@ class Net:
         ctx.check(capi.lib().pmx_net_create(ctx.h, max_batch, H0, W0, near, far, seed,
                                             ctypes.byref(h)))
         self.h = h
+        ctx.track(self)
         nt, nw = ctypes.c_size_t(), ctypes.c_size_t()
         ctx.check(capi.lib().pmx_net_num_params(h, ctypes.byref(nt), ctypes.byref(nw)))
         self.n_params, self.n_weights = nt.value, nw.value
