@@ -32,6 +32,7 @@
 #include "pmx_oracle.h"
 
 #include <atomic>
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -209,6 +210,10 @@ int pmxo_blur_taps(double sigma, int mode, double* taps) {
  * addressing of the texture reference defaults). */
 void pmxo_blur(const double* in, int width, int height, double sigma, int mode,
                double* out) {
+    if (mode == 0) { /* no pre-blur: the reference's own behaviour (the filter is never called) */
+        std::copy(in, in + (size_t)width * height, out);
+        return;
+    }
     double taps[45];
     const int tw = pmxo_blur_taps(sigma, mode, taps);
     const int fr = tw >> 1;

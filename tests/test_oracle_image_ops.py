@@ -83,3 +83,11 @@ def test_box_sums_constant_counts_in_bounds():
 def test_transpose_is_permutation(shape):
     img = np.arange(shape[0] * shape[1], dtype=np.float64).reshape(shape)
     assert np.array_equal(O.transpose(img), img.T)
+
+
+def test_no_blur_mode_is_the_identity():
+    """Blur mode 0 leaves the image as it is (the reference never calls its
+    filter, src/patchMatch.cu:150-205), for any sigma."""
+    img = np.random.default_rng(3).uniform(0, 1, (23, 31))
+    for sigma in (0.5, 1.0, 4.0):
+        assert np.array_equal(O.blur(img, sigma, 0), img)
