@@ -174,3 +174,32 @@ def test_batch_autotune(views, gpu_ctx):
     b.close()
     for e in views:
         e[3].set_sweep_waves(1)
+
+
+def test_batch_with_proposal(gpu_ctx):
+    """A learned proposal installed on one engine of the batch: that view
+    equals its standalone run with the same proposal, the others their
+    plain runs."""
+    _say("case: batch with a proposal on one view")
+    engs = _engines(gpu_ctx, shapes=SHAPES[:3])
+    spec, rp, sc, e0 = engs[0]
+    rng = np.random.default_rng(5)
+    prop = (sc.gt_depth * (1 + 0.03 * rng.uniform(-1, 1, sc.gt_depth.shape))).astype(np.float32)
+    prop[rng.uniform(size=prop.shape) > 0.7] = 0.0
+    spread = np.where(rng.uniform(size=prop.shape) < 0.5, 0.02 / sc.gt_depth, 0.0).astype(np.float32)
+    pd = capi.DeviceArray(gpu_ctx, prop.shape, np.float32).upload(prop)
+    sd = capi.DeviceArray(gpu_ctx, prop.shape, np.float32).upload(spread)
+    e0.set_proposal(pd.ptr, e0.W * 4, sd.ptr)
+    want = _standalone(engs)
+    b = capi.Batch(gpu_ctx, [e[3] for e in engs])
+    for graph in (False, True):
+        b.set_graph(graph)
+        b.prepare()
+        b.run()
+        _assert_same(engs, want, f"proposal, graph={graph}")
+    b.close()
+    e0.set_proposal(None, 0)
+    for e in engs:
+        e[3].close()
+    pd.close()
+    sd.close()
