@@ -279,11 +279,14 @@ def batch_bench(args):
         for e in engs:
             e.set_storage(capi.STORE_F16)
     b = capi.Batch(ctx, engs)
-    if args.no_autotune:
+    if args.batch_shape is not None:
+        b.set_launch(args.batch_shape, 0)
+        waves, wgs = b.launch()
+    elif args.no_autotune:
         waves, wgs = b.launch()
     else:
         waves, wgs, tune_ms = b.autotune()
-        log(f"autotuned: {waves} wave(s) per column strip, {wgs} workgroups ({tune_ms:.1f} ms)")
+        log(f"autotuned: launch shape {waves}, {wgs} workgroups ({tune_ms:.1f} ms)")
     b.set_graph(True)
     for _ in range(args.warmup):
         b.prepare()
@@ -316,7 +319,7 @@ def batch_bench(args):
         "data": "synthetic (16 seeded scenes, no dataset)",
         "config": {"workload": "c3_batch", "views": [f"{e[0].W}x{e[0].H}" for e in views],
                    "src_views": 10, "window": 15, "iterations": 4, "num_samples": 5,
-                   "graph": True, "waves_per_strip": waves, "workgroups": wgs},
+                   "graph": True, "launch_shape": waves, "workgroups": wgs},
         "gpu_ms_per_run": run_ms / args.steps,
     }
     if args.no_cpu_baseline:
@@ -455,6 +458,8 @@ def main():
     ap.add_argument("--fp16", action="store_true", help="FP16 texel storage")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched workload: keep the default launch shape")
+    ap.add_argument("--batch-shape", type=int, default=None,
+                    help="batched workload: this launch shape (1, 2 or 3), no autotuning")
     ap.add_argument("--batch-crop", default="128x96",
                     help="per-view crop of the batched workload's oracle check")
     args = ap.parse_args()

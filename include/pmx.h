@@ -259,12 +259,15 @@ int pmx_batch_run(pmx_batch* batch, float* ms);
 /* Replay from a hipGraph (default 1); the graph is rebuilt when the launch
  * shape, storage, iterations or proposal presence change. */
 int pmx_batch_set_graph(pmx_batch* batch, int enabled);
-/* Launch shape: waves per column strip (1 or 2) and persistent workgroups
- * (0 = every slot the device keeps resident, else at least 8: one per XCD
+/* Launch shape: waves per column strip (1 or 2, with every register the
+ * column code wants: two resident waves per SIMD), or 3 (one wave per strip
+ * at three resident waves per SIMD, 168 registers with a few spilled; window
+ * sizes 11 and 15, others run shape 1), and persistent workgroups (0 =
+ * every slot the device keeps resident, else at least 8: one per XCD
  * queue). */
 int pmx_batch_set_launch(pmx_batch* batch, int waves, int workgroups);
 int pmx_batch_get_launch(pmx_batch* batch, int* waves, int* workgroups);
-/* Autotuner: times full runs (prepare + run) for waves 1 and 2 at all,
+/* Autotuner: times full runs (prepare + run) for shapes 1, 2 and 3 at all,
  * 3/4 and 1/2 of the resident workgroups, keeps the fastest and returns it.
  * The engines are left run; prepare them again before pmx_batch_run. */
 int pmx_batch_autotune(pmx_batch* batch, int* waves, int* workgroups, float* ms);

@@ -1276,7 +1276,7 @@ int pmx_engine_reset_stats(pmx_engine* e) {
 struct pmx_batch {
     pmx_ctx* ctx = nullptr;
     std::vector<pmx_engine*> engs;
-    int waves = 1;            /* waves per column strip (1 or 2) */
+    int waves = 1;            /* launch shape: waves per column strip (1 or 2), or 3 */
     int grid = 0;             /* persistent workgroups; 0 = every resident slot */
     bool use_graph = true;
     int steps = 0;            /* sweep steps of one run */
@@ -1563,8 +1563,8 @@ int pmx_batch_set_graph(pmx_batch* b, int enabled) {
 
 int pmx_batch_set_launch(pmx_batch* b, int waves, int workgroups) {
     PMXCHK(check_batch(b));
-    if (waves != 1 && waves != 2)
-        return fail(b->ctx, PMX_ERR_ARG, "waves per column strip must be 1 or 2 (got %d)", waves);
+    if (waves < 1 || waves > 3)
+        return fail(b->ctx, PMX_ERR_ARG, "launch shape must be 1, 2 or 3 (got %d)", waves);
     /* every one of the 8 XCD queues needs a workgroup b with b % 8 == x */
     if (workgroups != 0 && workgroups < 8)
         return fail(b->ctx, PMX_ERR_ARG, "workgroups must be 0 (auto) or at least 8 (got %d)",
@@ -1606,7 +1606,7 @@ int pmx_batch_autotune(pmx_batch* b, int* waves, int* workgroups, float* ms) {
     float best = 0.0f;
     int bw = 1, bg = 0;
     int rc = PMX_OK;
-    for (int w = 1; w <= 2 && rc == PMX_OK; w++) {
+    for (int w = 1; w <= 3 && rc == PMX_OK; w++) {
         b->waves = w;
         b->grid = 0;
         int full = 0;

@@ -669,7 +669,7 @@ __device__ __forceinline__ float ncc_finish(float A, float B, float C, float K,
  * sums to part[t * NW + slice] and the caller combines them after a
  * barrier (combine_pass).  Returns k.
  */
-template <int G, int WN, int NW, class SB>
+template <int G, int WN, int NW, int NB = 0, class SB>
 __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __restrict__ Hm,
                                          const SB& sb, const float* __restrict__ win,
                                          int wave, int lane, int nt, const int* tview,
@@ -679,8 +679,8 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
     constexpr int NGT = NG * NW;
     /* tap rows in flight per lane: a whole column with one wave per
      * workgroup (192 VGPRs, 2 waves per SIMD); 8 rows with more waves,
-     * which must fit 512 / (2 NW) VGPRs */
-    constexpr int kNB = NW == 1 ? 16 : 8;
+     * which must fit 512 / (2 NW) VGPRs; NB > 0 sets it */
+    constexpr int kNB = NB > 0 ? NB : (NW == 1 ? 16 : 8);
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
     int k = 1;
     while (2 * k * nt <= NGT) k *= 2;
@@ -1051,7 +1051,7 @@ __device__ __forceinline__ float proposal_draw(float depth, float spread, int x,
  * (the reference recomputes the same value).  Wave 0 alone writes the
  * column's outputs.
  */
-template <int G, int PF, int WN, int NW, bool H16>
+template <int G, int PF, int WN, int NW, bool H16, int NB = 0>
 __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c) {
     extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
@@ -1160,7 +1160,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         for (int t0 = 0; t0 < nt; t0 += NGT) {
             const int ntp = min(NGT, nt - t0);
             float4* pp = part + (pb & 1) * 64 * NW;
-            const int kk = eval_pass<G, WN, NW>(a, Hs, sb, win, wave, lane, ntp, tview + t0,
+            const int kk = eval_pass<G, WN, NW, NB>(a, Hs, sb, win, wave, lane, ntp, tview + t0,
                                                 tdep + t0, tout + t0, pp, pc);
             __syncthreads();
             if (kk > NG) combine_pass<NW>(lane, ntp, kk / NG, pp, tout + t0, pc);
@@ -1356,12 +1356,20 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
  *   queue[x]          next free slot of queue x (zero before the launch)
  * Workgroup b runs on XCD b % 8 (round-robin dispatch), which only decides
  * locality: every workgroup drains its queue, so every column is processed
- * whatever the placement.
+ * whatever the placement.  WPE is the register budget in resident waves per
+ * SIMD (512 / WPE registers per lane): 2 * NW keeps every register the
+ * column code wants; 3 with one wave per strip puts a third column in
+ * flight on each SIMD, which a batch (with far more columns than slots)
+ * can fill.  In 168 registers a whole window column of taps in flight
+ * spills, so those taps go out in batches of NB rows: 8 with FP16 texels,
+ * 5 with FP32 (the fastest on the 16-view 10-source workload; 4 to 16
+ * measured).
  */
-template <int G, int PF, int WN, int NW, bool H16>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2 * NW > 8 ? 8 : 2 * NW)))
+template <int G, int PF, int WN, int NW, bool H16, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE)))
 void pmx_sweep_batch_kernel(const PmxSweepArgs* __restrict__ tab, const int* __restrict__ qstart,
                             const int* __restrict__ qcol, int n, unsigned* __restrict__ queue) {
+    constexpr int NB = (WPE == 3 && NW == 1) ? (H16 ? 8 : 5) : 0;
     __shared__ int s_slot;
     const int x = (int)(blockIdx.x & 7u);
     const int* __restrict__ st = qstart + x * (n + 1);
@@ -1374,7 +1382,7 @@ void pmx_sweep_batch_kernel(const PmxSweepArgs* __restrict__ tab, const int* __r
         if (j >= total) break;
         int v = 0;
         while (st[v + 1] <= j) v++;
-        sweep_column<G, PF, WN, NW, H16>(tab[v], qcol[x * n + v] + (j - st[v]));
+        sweep_column<G, PF, WN, NW, H16, NB>(tab[v], qcol[x * n + v] + (j - st[v]));
         __syncthreads();
     }
 }
@@ -1515,38 +1523,46 @@ hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
                            sh, s, a);
     return hipGetLastError();
 }
-template <int G, int PF, int WN, int NW>
+template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW)>
 hipError_t launch_sweep_batch_t(hipStream_t s, const PmxBatchLaunch& b) {
     const int Wn = 2 * b.hw + 1;
     const size_t sh = sizeof(double) * b.max_V * 18 + sizeof(float4) * 2 * 64 * NW +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     if (b.h16)
-        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, true>), dim3(b.grid),
+        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, true, WPE>), dim3(b.grid),
                            dim3(64 * NW), sh, s, b.tab, b.qstart, b.qcol, b.n, b.queue);
     else
-        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, false>), dim3(b.grid),
+        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, false, WPE>), dim3(b.grid),
                            dim3(64 * NW), sh, s, b.tab, b.qstart, b.qcol, b.n, b.queue);
     return hipGetLastError();
 }
 
-template <int G, int PF, int WN, int NW>
+template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW)>
 hipError_t occupancy_batch_t(const PmxBatchLaunch& b, int* per_cu) {
     const int Wn = 2 * b.hw + 1;
     const size_t sh = sizeof(double) * b.max_V * 18 + sizeof(float4) * 2 * 64 * NW +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     if (b.h16)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, true>, 64 * NW, sh);
+            per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, true, WPE>, 64 * NW, sh);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, false>, 64 * NW, sh);
+        per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, false, WPE>, 64 * NW, sh);
 }
 } // namespace
 
 /* Batched sweeps: the same shapes as single-view sweeps with one or two
- * waves per column strip.  per_cu != NULL: report the resident workgroups
- * per CU instead of launching. */
+ * waves per column strip; shape 3 is one wave per strip at three resident
+ * waves per SIMD (the configured window sizes only; others run shape 1).
+ * per_cu != NULL: report the resident workgroups per CU instead of
+ * launching. */
 hipError_t pmx_launch_sweep_batch(hipStream_t s, const PmxBatchLaunch& b, int* per_cu) {
     const int Wn = 2 * b.hw + 1;
+    if (b.waves == 3 && Wn == 15)
+        return per_cu ? occupancy_batch_t<16, 4, 15, 1, 3>(b, per_cu)
+                      : launch_sweep_batch_t<16, 4, 15, 1, 3>(s, b);
+    if (b.waves == 3 && Wn == 11)
+        return per_cu ? occupancy_batch_t<16, 2, 11, 1, 3>(b, per_cu)
+                      : launch_sweep_batch_t<16, 2, 11, 1, 3>(s, b);
 #define PMX_BATCH_NW(G, PF, WN)                                                              \
     if (b.waves == 2)                                                                         \
         return per_cu ? occupancy_batch_t<G, PF, WN, 2>(b, per_cu)                           \

@@ -99,6 +99,32 @@ def test_batch_launch_shapes(views, gpu_ctx, waves, workgroups):
         e[3].set_sweep_waves(1)
 
 
+@pytest.mark.parametrize("hw,storage", [(7, capi.STORE_F32), (7, capi.STORE_F16),
+                                        (5, capi.STORE_F32)])
+def test_batch_three_waves_per_simd(gpu_ctx, hw, storage):
+    """Launch shape 3 (one wave per strip, a register budget of three waves
+    per SIMD) at the window sizes it is built for: the same column code as
+    one wave per strip, so the same results bit for bit."""
+    _say(f"case: batch shape 3, window {2 * hw + 1}, storage {storage}")
+    engs = []
+    for W, H, V, seed in [(64, 48, 3, 21), (45, 61, 2, 22), (70, 33, 4, 23)]:
+        spec, rp = configs.tiny(W=W, H=H, V=V, hw=hw, S=2, iters=2, seed=seed)
+        sc, eng = make_engine(gpu_ctx, spec, rp, prepare=False)
+        eng.set_storage(storage)
+        engs.append((spec, rp, sc, eng))
+    want = _standalone(engs, 1)
+    b = capi.Batch(gpu_ctx, [e[3] for e in engs])
+    for wgs in (0, 24):
+        b.set_launch(3, wgs)
+        assert b.launch()[0] == 3
+        b.prepare()
+        b.run()
+        _assert_same(engs, want, f"shape 3, {wgs} workgroups")
+    b.close()
+    for e in engs:
+        e[3].close()
+
+
 def test_batch_fp16(gpu_ctx):
     _say("case: batch with FP16 storage vs standalone FP16 runs")
     engs = _engines(gpu_ctx, capi.STORE_F16, SHAPES[:3])
@@ -152,7 +178,9 @@ def test_batch_errors(gpu_ctx):
     with pytest.raises(capi.PmxError):
         b.run()                                     # not freshly prepared
     with pytest.raises(capi.PmxError):
-        b.set_launch(3, 0)                          # 1 or 2 waves per strip
+        b.set_launch(4, 0)                          # shapes 1, 2 and 3 only
+    with pytest.raises(capi.PmxError):
+        b.set_launch(0, 0)
     with pytest.raises(capi.PmxError):
         b.set_launch(1, 4)                          # fewer workgroups than XCD queues
     b.close()
@@ -165,9 +193,9 @@ def test_batch_autotune(views, gpu_ctx):
     b = capi.Batch(gpu_ctx, [e[3] for e in views])
     w, g, ms = b.autotune()
     _say(f"  chosen waves={w} workgroups={g}: {ms:.3f} ms")
-    assert w in (1, 2) and g >= 8 and ms > 0
+    assert w in (1, 2, 3) and g >= 8 and ms > 0
     assert b.launch() == (w, g)
-    want = _standalone(views, w)
+    want = _standalone(views, 1 if w == 3 else w)
     b.prepare()
     b.run()
     _assert_same(views, want, "after autotune")
