@@ -136,6 +136,12 @@ class Server:
             return None
         b = capi.Batch(ctx, [e for _, _, e in admitted])
         b.set_graph(False)
+        # launch shape 3 (a third resident wave per SIMD) pays once the group
+        # has more column strips than shape 1 keeps resident; with fewer its
+        # shorter tap batches only add latency
+        strips = sum(min(self.dims(req)) for req, _, _ in admitted)
+        if strips > b.launch()[1]:
+            b.set_launch(3, 0)
         b.prepare()
         if self.weights is not None:
             for req, _, e in admitted:
