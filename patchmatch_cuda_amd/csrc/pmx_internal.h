@@ -84,7 +84,7 @@ struct PmxBatchLaunch {
     int n;
     unsigned* queue;         /* 8 counters, zero before the launch */
     int grid;                /* resident workgroups */
-    int waves;               /* 1 or 2 per column strip */
+    int waves;               /* launch shape: 1 or 2 waves per column strip, or 3 */
     int hw, max_V, h16;
 };
 
