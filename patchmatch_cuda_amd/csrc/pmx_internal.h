@@ -147,7 +147,7 @@ struct pmx_engine {
     unsigned long long cost_all_evals = 0;   /* closed form, counted on the host */
     unsigned long long cost_all_samples = 0;
     bool prepared = false;
-    int sweep_waves = 1;        /* wavefronts per sweep column (1, 2 or 4; 0 = automatic) */
+    int sweep_waves = 0;        /* wavefronts per sweep column (1, 2 or 4; 0 = automatic) */
     int num_cus = 0;            /* compute units of the device (queried once) */
 
     /* timing */

@@ -205,13 +205,14 @@ def test_full_run_c1_depth_agreement(c1):
     assert agree >= 0.99
 
 
-def test_full_run_c1_auto_waves(c1):
-    """The automatic launch shape (two waves per column at three waves per
-    SIMD wherever the columns fit) meets the same bar, and its teacher-forced
-    sweep agrees decision for decision."""
+def test_full_run_c1_one_wave(c1):
+    """One wave per column everywhere (the default automatic shape runs two
+    waves per column at three waves per SIMD wherever the columns fit, as on
+    every sweep of this 320x240 view) meets the same bar, and its
+    teacher-forced sweep agrees decision for decision."""
     spec, rp, sc, eng = c1
-    _say("case: full run c1, automatic sweep waves")
-    eng.set_sweep_waves(0)
+    _say("case: full run c1, one wave per sweep column")
+    eng.set_sweep_waves(1)
     try:
         eng.prepare()
         eng.cost_all_views(rotated=True)
@@ -235,4 +236,4 @@ def test_full_run_c1_auto_waves(c1):
         _say(f"  within 1% of oracle: {agree:.4f}")
         assert agree >= 0.99
     finally:
-        eng.set_sweep_waves(1)
+        eng.set_sweep_waves(0)

@@ -47,6 +47,7 @@ def test_served_depths_equal_standalone_runs(gpu_ctx):
         sc = pool[r.size]
         eng = capi.Engine(gpu_ctx, rp, sc.ref, list(sc.srcs[:r.n_src]), sc.cams[:r.n_src + 1])
         eng.set_storage(capi.STORE_F16)
+        eng.set_sweep_waves(1)      # the batched launch runs one wave per strip
         eng.prepare()
         eng.run()
         want = eng.depth()
