@@ -830,10 +830,25 @@ __device__ __forceinline__ void load_homographies(const PmxSweepArgs& a, double*
     for (int i = (int)threadIdx.x; i < a.V * 18; i += (int)blockDim.x) Hs[i] = a.Hm[i];
 }
 
+/* Pixel tiles of the cost-all launch: kCostTW x kCostTH pixels per tile,
+ * tiles row-major over the image, pixels row-major inside a tile. */
+constexpr int kCostTW = 64, kCostTH = 16;
+
 /* computeAllCostGivenDepth (src/patchMatch.cu:775-816): one wavefront per
- * pixel, G-lane groups over views.  Writes the row-major state. */
+ * pixel, G-lane groups over views.  Writes the row-major state.  Workgroups
+ * walk the image in 64 x 16 pixel tiles, and each XCD takes a contiguous
+ * run of tiles: the windows of a tile's 16 rows overlap in the sources, so
+ * an XCD's L2 serves most taps (in row-major order the 15-row reuse spans
+ * whole image rows of every view, more than the L2 holds). */
 template <int G, int PF, int WN, bool H16>
 __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
+    const int pid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int tiles_x = (a.W + kCostTW - 1) / kCostTW;
+    const int tile = pid / (kCostTW * kCostTH), in_tile = pid - tile * (kCostTW * kCostTH);
+    const int x = (tile % tiles_x) * kCostTW + in_tile % kCostTW;
+    const int y = (tile / tiles_x) * kCostTH + in_tile / kCostTW;
+    /* the whole workgroup is one pixel: it leaves together, before any barrier */
+    if (x >= a.W || y >= a.H) return;
     extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
     double* Hs = smem_d;                               /* V x 18 */
@@ -846,10 +861,6 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
     constexpr int NG = kLanes / G;
     for (int i = Wn * Wn + lane; i < 2 * Wn * Wn; i += kLanes) win[i] = 0.0f;
     load_homographies(a, Hs);
-    /* contiguous pixel runs per XCD: neighbouring pixels read neighbouring
-     * source footprints, which then stay in that XCD's L2 */
-    const int pid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int x = pid % a.W, y = pid / a.W;
     const float inv_wn = 1.0f / (float)Wn;
     const SrcBuf<H16> sb = src_buf<H16>(a);
     PixelCtx pc;
@@ -1489,7 +1500,8 @@ template <int G, int PF, int WN>
 hipError_t launch_cost_all_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
     const size_t sh = sizeof(double) * a.V * 18 + sizeof(float) * (2 * Wn * Wn + 3 * 64);
-    const dim3 grid((unsigned)((long)a.W * a.H));
+    const long tiles = (long)((a.W + kCostTW - 1) / kCostTW) * ((a.H + kCostTH - 1) / kCostTH);
+    const dim3 grid((unsigned)(tiles * kCostTW * kCostTH));
     if (a.src_h16)
         hipLaunchKernelGGL((pmx_cost_all_kernel<G, PF, WN, true>), grid, dim3(64), sh, s, a);
     else
