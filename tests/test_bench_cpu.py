@@ -20,3 +20,17 @@ def test_help_lists_the_workloads_and_legs():
     assert r.returncode == 0
     for flag in ("--steps", "--warmup", "--workload", "--fp16", "--no-learned", "--tau"):
         assert flag in r.stdout
+    assert "--batch-shape" in r.stdout
+
+
+def test_profiled_traffic_is_read_from_the_committed_summary():
+    """The roofline's traffic figure comes from the rocprof summary of the
+    same workload: it parses, and it is at least the bytes of sweep state
+    the algorithm itself moves per launch ((16 V + 20) B per pixel)."""
+    sys.path.insert(0, ROOT)
+    import bench
+    traffic, src = bench.profiled_traffic("c2_realistic")
+    assert src == os.path.join("profiles", "sweep_kernel_c2_realistic.md")
+    state = (16 * 10 + 20) * 2048 * 1536
+    assert state < traffic < 20 * state
+    assert bench.profiled_traffic("no_such_workload") == (None, None)
