@@ -104,7 +104,7 @@ def build_drop_in():
 
 def build_oracle():
     oracle_dir = os.path.join(ROOT, "oracle")
-    _run(["make", "-s", "-C", oracle_dir])
+    _run(["make", "-s", "-B", "-C", oracle_dir])
 
 
 def build_all():
