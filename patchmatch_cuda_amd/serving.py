@@ -91,6 +91,7 @@ class Server:
         self.resident = 0
         self.keep_results = False
         self.results = {}
+        self.launch_shapes = []   # batch launch shape of every group, in order
 
     def dims(self, req):
         spec = self.pool[req.size].spec
@@ -142,6 +143,7 @@ class Server:
         strips = sum(min(self.dims(req)) for req, _, _ in admitted)
         if strips > b.launch()[1]:
             b.set_launch(3, 0)
+        self.launch_shapes.append(b.launch()[0])
         b.prepare()
         if self.weights is not None:
             for req, _, e in admitted:

@@ -57,6 +57,36 @@ def test_served_depths_equal_standalone_runs(gpu_ctx):
     ctx2.close()
 
 
+def test_large_group_runs_shape_3(gpu_ctx):
+    """Four 1024x768 requests arriving together form one group with more
+    column strips (4 x 768) than launch shape 1 keeps resident, so the group
+    runs shape 3; every served depth map still equals a standalone run of
+    the same request with one wave per column, bit for bit."""
+    _say("case: serving group of four 1024x768 requests (launch shape 3)")
+    pool = []
+    for i in range(len(serving.SIZES)):
+        spec, _ = configs.tiny(W=1024, H=768, V=4, hw=7, S=2, iters=1, seed=60 + i, planes=4)
+        pool.append(scene.render(spec))
+    _, rp = configs.tiny(hw=7, S=2, iters=1)
+    stream = [serving.Request(i, 0.0, i, 3 + i % 2) for i in range(4)]
+    srv = serving.Server([gpu_ctx], rp, pool, budget_bytes=2 ** 40)
+    srv.keep_results = True
+    res = srv.run(stream, log=_say)
+    _say(f"  {res['requests']} requests in {res['batches']} group(s), shapes {srv.launch_shapes}")
+    assert res["requests"] == 4 and res["batches"] == 1
+    assert srv.launch_shapes == [3]
+    for r in stream:
+        sc = pool[r.size]
+        eng = capi.Engine(gpu_ctx, rp, sc.ref, list(sc.srcs[:r.n_src]), sc.cams[:r.n_src + 1])
+        eng.set_storage(capi.STORE_F16)
+        eng.set_sweep_waves(1)
+        eng.prepare()
+        eng.run()
+        want = eng.depth()
+        eng.close()
+        assert np.array_equal(srv.results[r.rid], want), r
+
+
 def test_footprint_matches_device_allocation(gpu_ctx):
     """pmx_engine_footprint (the admission measure) against the device memory
     an engine actually takes: hipMemGetInfo before and after creating an
