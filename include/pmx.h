@@ -245,8 +245,9 @@ int pmx_engine_get_storage(pmx_engine* eng, int* storage);
  * schedule (cost of all views, sweeps, transposes) is captured once in a
  * hipGraph and replayed.  The engines must share half_window, num_samples,
  * num_iterations and storage; their sizes and view counts may differ.
- * Each engine's results equal a pmx_engine_run of that engine alone with
- * the same launch shape (waves per column strip).  The batch borrows the
+ * Each engine's results equal, bit for bit, a pmx_engine_run of that engine
+ * alone with the same waves per column (launch shapes 1 and 3: one wave,
+ * pmx_engine_set_sweep_waves(eng, 1); shape 2: two).  The batch borrows the
  * engines: destroy it before any of them. */
 typedef struct pmx_batch pmx_batch;
 int pmx_batch_create(pmx_ctx* ctx, pmx_engine* const* engines, int num_views, pmx_batch** out);
