@@ -91,7 +91,7 @@ class Server:
         self.resident = 0
         self.keep_results = False
         self.results = {}
-        self.launch_shapes = []   # batch launch shape of every group, in order
+        self.launch_shapes = []   # batch launch shape of every group of the last run
 
     def dims(self, req):
         spec = self.pool[req.size].spec
@@ -167,6 +167,7 @@ class Server:
         return px
 
     def run(self, stream, log=print):
+        self.launch_shapes = []
         pending = deque(stream)
         waiting = deque()
         lat, done_px, sizes = [], 0, []
@@ -216,6 +217,8 @@ class Server:
                           "max": float(max(lat)) if lat else float("nan")},
             "batches": len(sizes),
             "mean_batch": float(np.mean(sizes)) if sizes else 0.0,
+            "groups_by_launch_shape": {str(k): self.launch_shapes.count(k)
+                                       for k in sorted(set(self.launch_shapes))},
         }
 
 

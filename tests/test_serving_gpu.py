@@ -75,6 +75,7 @@ def test_large_group_runs_shape_3(gpu_ctx):
     _say(f"  {res['requests']} requests in {res['batches']} group(s), shapes {srv.launch_shapes}")
     assert res["requests"] == 4 and res["batches"] == 1
     assert srv.launch_shapes == [3]
+    assert res["groups_by_launch_shape"] == {"3": 1}
     for r in stream:
         sc = pool[r.size]
         eng = capi.Engine(gpu_ctx, rp, sc.ref, list(sc.srcs[:r.n_src]), sc.cams[:r.n_src + 1])
