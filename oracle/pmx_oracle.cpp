@@ -563,6 +563,9 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
             if (trace->cost_margin) trace->cost_margin[i] = INFINITY;
             if (trace->choice) trace->choice[i] = 255;
             if (trace->near_clamp) trace->near_clamp[i] = 0;
+            if (trace->rand_depth) trace->rand_depth[i] = NAN;
+            for (int v = 0; trace->cdf && v < V; v++) trace->cdf[i * V + v] = NAN;
+            for (int q = 0; trace->cand_cost && q < 4; q++) trace->cand_cost[i * 4 + q] = NAN;
         }
     }
 
@@ -675,6 +678,10 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
                 if (trace->cost_margin) trace->cost_margin[pi] = cm;
                 if (trace->choice) trace->choice[pi] = (uint8_t)idx;
                 if (trace->near_clamp) trace->near_clamp[pi] = clamp_flag ? 1 : 0;
+                if (trace->rand_depth) trace->rand_depth[pi] = d_rand;
+                for (int v = 0; trace->cdf && v < V; v++) trace->cdf[pi * V + v] = cdf[v];
+                for (int q = 0; trace->cand_cost && q < 4; q++)
+                    trace->cand_cost[pi * 4 + q] = q < ncand ? cost3[q] : NAN;
             }
             m = m_prev;
             load_costs(r);

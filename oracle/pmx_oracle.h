@@ -78,6 +78,10 @@ typedef struct pmxo_trace {
     double*  cost_margin;  /* min |c_best - c_k| over candidates k with another depth */
     uint8_t* choice;       /* 0 former, 1 current, 2 random, 3 proposal; 255 = row not processed */
     uint8_t* near_clamp;   /* 1 if any variance term came within 0.1% of 1e-5 */
+    /* Optional (NULL = not recorded); rows not processed hold NaN. */
+    double*  cdf;          /* Ws*Hs*V: the normalised view CDF the row's draws read, [r][c][v] */
+    double*  rand_depth;   /* Ws*Hs: the row's random depth hypothesis */
+    double*  cand_cost;    /* Ws*Hs*4: averaged costs of former, current, random, proposal */
 } pmxo_trace;
 
 typedef struct pmxo_problem pmxo_problem;
