@@ -1032,6 +1032,10 @@ int pmx_engine_propose(pmx_engine* e, pmx_net* net, int view0, int view1, float 
     PMXCHK(check_engine(e));
     PMXCHK(require_prepared(e));
     if (!net) return fail(e->ctx, PMX_ERR_ARG, "net is NULL");
+    /* the cost volume runs on the engine's stream and the network on its
+     * own context's: one stream orders them only when both are the same */
+    if (pmx_net_context(net) != e->ctx)
+        return fail(e->ctx, PMX_ERR_ARG, "the network was created on another context");
     pmx_ctx* ctx = e->ctx;
     const int W0 = (e->W + 3) / 4, H0 = (e->H + 3) / 4;
     if (view0 < 0 || view1 < 0) volume_views(e, &view0, &view1);

@@ -219,6 +219,9 @@ hipError_t pmx_launch_proposal(hipStream_t s, const float* Hf, int C, int H0, in
 hipError_t pmx_launch_target(hipStream_t s, const float* depth, int ld, int W, int H, int W0, int H0,
                              float* tgt);
 
+/* The context a proposal network was created on (pmx_net.hip). */
+pmx_ctx* pmx_net_context(const pmx_net* net);
+
 /* Error reporting shared by the C ABI sources: records the message for
  * pmx_last_error and returns code. */
 int pmx_fail(pmx_ctx* ctx, int code, const char* fmt, ...);

@@ -287,6 +287,8 @@ int copy_sums(pmx_net* net, float* loss) {
 
 }  // namespace
 
+pmx_ctx* pmx_net_context(const pmx_net* net) { return net ? net->ctx : nullptr; }
+
 extern "C" {
 
 int pmx_net_create(pmx_ctx* ctx, int max_batch, int H0, int W0, float depth_near, float depth_far,

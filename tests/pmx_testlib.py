@@ -81,3 +81,45 @@ def within(a, b, rel=0.01):
 
 
 TINY = configs.tiny()
+
+
+def proposal_planes(ctx, eng, net, tau):
+    """The network's proposal for a prepared engine as host planes (depth,
+    0 = none; spread in inverse depth), computed by the same kernels as
+    pmx_engine_propose: cost volume on the engine's two volume views, the
+    network's forward pass, the upsampled proposal."""
+    from patchmatch_cuda_amd import proposal
+
+    W, H = eng.W, eng.H
+    W0, H0 = proposal.low_res(W, H)
+    x = capi.DeviceArray(ctx, (1, H0, W0, proposal.IN_C), np.uint16)
+    d = capi.DeviceArray(ctx, (H, W), np.float32)
+    sp = capi.DeviceArray(ctx, (H, W), np.float32)
+    eng.cost_volume(*eng.volume_views(), x.ptr)
+    net.forward(x)
+    net.propose(W, H, tau, d, sp)
+    out = d.download(), sp.download()
+    for a in (x, d, sp):
+        a.close()
+    return out
+
+
+def oracle_problem(ref, srcs, H1H2, cams0, rp):
+    """Oracle problem on given (pre-blurred) images and homographies."""
+    return O.Problem(ref, srcs, H1H2, rp.half_window, f32(rp.depth_near), f32(rp.depth_far),
+                     f32(rp.spm_alpha), O.inv_k4(cams0[:9]), f32(rp.orientation_x),
+                     f32(rp.orientation_z))
+
+
+def centred_crop(sc, w, h):
+    from patchmatch_cuda_amd import scene as _scene
+
+    x0, y0 = (sc.spec.W - w) // 2, (sc.spec.H - h) // 2
+    return (x0, y0), _scene.crop_reference(sc, x0, y0, w, h)
+
+
+def ncc_diff(cg, co):
+    """|GPU - oracle| of costs; costs outside [0, 2] (only where the
+    reference's tile-index quirk pairs shifted reference values) relatively."""
+    legit = np.abs(co) <= 2.0
+    return np.where(legit, np.abs(cg - co), np.abs(cg - co) / np.maximum(np.abs(co), 1.0))
