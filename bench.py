@@ -117,16 +117,22 @@ def crop_problem(sc, sample):
     return (x0, y0, w, h), scene.crop_reference(sc, x0, y0, w, h)
 
 
-def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau, seeding):
+def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, parity_crop, steps, warmup, tau,
+                seeding):
     """Learned-proposal PatchMatch against plain PatchMatch.
 
-    Sweeps needed: on the centred crop, each method runs 1..iterations
-    iterations and its final depth is compared with the converged FP64
-    oracle (plain method, all iterations, d_oracle); a method's iteration
-    count is the smallest whose depth has >= 99% of pixels within 1%.
-    The learned method is then timed on the full view at its count (a step
-    = prepare + cost volume + network + proposal + run), and its parity is
-    checked: at that count, the GPU's learned run on the crop against the
+    Sweeps needed: on the view the accuracy oracle covered (sample None: the
+    whole view; else its centred crop), each method runs 1..iterations
+    iterations, and its final depth is compared with the converged FP64
+    oracle (plain method, all iterations, d_oracle) and with the ground
+    truth.  A method's count is the smallest whose depth has >= 99% of
+    pixels within 1% of the oracle; the ground-truth count is the smallest
+    that comes within 0.1 points of the plain method's converged share of
+    pixels within 1% of the ground truth (a method that converges to a
+    different but equally good solution meets the second, not the first).
+    The learned method is timed on the full view at its count (a step =
+    prepare + cost volume + network + proposal + run), and its parity is
+    checked on the centred parity_crop: the GPU's learned run against the
     FP64 oracle running the same learned algorithm (same proposal planes,
     generator states and initial depth)."""
     from oracle import pmx_oracle as O
@@ -135,39 +141,66 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau,
     W0, H0 = proposal.low_res(eng.W, eng.H)
     net = proposal.Net(ctx, 1, H0, W0, rp.depth_near, rp.depth_far)
     net.load(weights)
-    (x0, y0, w, h), (ref, srcs, cams, gt) = crop_problem(sc, sample)
+    eng.set_proposal_seeding(seeding)
     eng.prepare()
     prop, spread = proposal_planes(ctx, eng, net, tau)
-    prop = np.ascontiguousarray(prop[y0:y0 + h, x0:x0 + w])
-    spread = np.ascontiguousarray(spread[y0:y0 + h, x0:x0 + w])
-    ceng = capi.Engine(ctx, rp, ref, list(srcs), cams)
-    ceng.set_proposal_seeding(seeding)
-    eng.set_proposal_seeding(seeding)
-    pd = capi.DeviceArray(ctx, (h, w), np.float32).upload(prop)
-    sd = capi.DeviceArray(ctx, (h, w), np.float32).upload(spread)
+    eng.set_proposal(None, 0)
+    if sample is None:
+        seng, gt, spd, ssd = eng, sc.gt_depth, None, None
+        where = f"whole {eng.W}x{eng.H} view"
+    else:
+        (x0, y0, w, h), (ref, srcs, cams, gt) = crop_problem(sc, sample)
+        seng = capi.Engine(ctx, rp, ref, list(srcs), cams)
+        seng.set_proposal_seeding(seeding)
+        spd = capi.DeviceArray(ctx, (h, w), np.float32).upload(
+            np.ascontiguousarray(prop[y0:y0 + h, x0:x0 + w]))
+        ssd = capi.DeviceArray(ctx, (h, w), np.float32).upload(
+            np.ascontiguousarray(spread[y0:y0 + h, x0:x0 + w]))
+        where = f"centred {w}x{h} crop"
     agree = lambda d: float(np.mean(np.abs(d - d_oracle) <= 0.01 * np.abs(d_oracle)))  # noqa
     on_gt = lambda d: float(np.mean(np.abs(d - gt) <= 0.01 * gt))  # noqa: E731
     study = {"plain": {}, "learned": {}}
-    need = {"plain": None, "learned": None}
     for k in range(1, rp.num_iterations + 1):
-        ceng.set_num_iterations(k)
+        seng.set_num_iterations(k)
         for method in ("plain", "learned"):
-            ceng.prepare()
+            seng.prepare()
             if method == "learned":
-                ceng.set_proposal(pd.ptr, w * 4, sd.ptr)
-            ceng.run()
-            d = ceng.depth().astype(np.float64)
-            ceng.set_proposal(None, 0)
-            a = agree(d)
-            study[method][k] = {"within_1pct_of_converged_oracle": a,
+                if spd is None:
+                    seng.propose(net, tau=tau)
+                else:
+                    seng.set_proposal(spd.ptr, spd.shape[1] * 4, ssd.ptr)
+            seng.run()
+            d = seng.depth().astype(np.float64)
+            seng.set_proposal(None, 0)
+            study[method][k] = {"sweeps": 4 * k,
+                                "within_1pct_of_converged_oracle": agree(d),
                                 "within_1pct_of_ground_truth": on_gt(d)}
-            if need[method] is None and a >= 0.99:
-                need[method] = k
-        log(f"crop, {k} iteration(s): plain {study['plain'][k]['within_1pct_of_converged_oracle']:.4f}"
-            f", learned {study['learned'][k]['within_1pct_of_converged_oracle']:.4f} of depths "
-            f"within 1% of the converged oracle")
+        log(f"{where}, {k} iteration(s): plain {study['plain'][k]['within_1pct_of_converged_oracle']:.4f}"
+            f" / {study['plain'][k]['within_1pct_of_ground_truth']:.4f}, learned "
+            f"{study['learned'][k]['within_1pct_of_converged_oracle']:.4f} / "
+            f"{study['learned'][k]['within_1pct_of_ground_truth']:.4f} of depths within 1% of the "
+            f"converged oracle / of the ground truth")
+    seng.set_num_iterations(rp.num_iterations)
+    if spd is not None:
+        seng.close()
+        spd.close()
+        ssd.close()
+    gt_final = study["plain"][rp.num_iterations]["within_1pct_of_ground_truth"]
+    need, need_gt = {}, {}
+    for m in ("plain", "learned"):
+        need[m] = next((k for k in sorted(study[m])
+                        if study[m][k]["within_1pct_of_converged_oracle"] >= 0.99), None)
+        need_gt[m] = next((k for k in sorted(study[m])
+                           if study[m][k]["within_1pct_of_ground_truth"] >= gt_final - 0.001), None)
     k = need["learned"] or rp.num_iterations
-    # parity of the learned path at k
+    # parity of the learned path at k, on the parity crop
+    (x0, y0, w, h), (ref, srcs, cams, gtc) = crop_problem(sc, parity_crop)
+    pc = np.ascontiguousarray(prop[y0:y0 + h, x0:x0 + w])
+    sc_ = np.ascontiguousarray(spread[y0:y0 + h, x0:x0 + w])
+    ceng = capi.Engine(ctx, rp, ref, list(srcs), cams)
+    ceng.set_proposal_seeding(seeding)
+    pd = capi.DeviceArray(ctx, (h, w), np.float32).upload(pc)
+    sd = capi.DeviceArray(ctx, (h, w), np.float32).upload(sc_)
     ceng.set_num_iterations(k)
     ceng.prepare()
     ceng.set_proposal(pd.ptr, w * 4, sd.ptr)
@@ -179,14 +212,15 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau,
     pb = O.Problem(ref_b, srcs_b, ceng.homographies(), rp.half_window, f(rp.depth_near),
                    f(rp.depth_far), f(rp.spm_alpha), O.inv_k4(cams[0, :9]),
                    f(rp.orientation_x), f(rp.orientation_z))
-    pb.set_proposal(prop.astype(np.float64), spread.astype(np.float64))
+    pb.set_proposal(pc.astype(np.float64), sc_.astype(np.float64))
     do, _, _ = pb.run(d0.astype(np.float64), st0.copy(), rp.num_samples, k, record_ids=False)
     parity = float(np.mean(np.abs(dg - do) <= 0.01 * np.abs(do)))
-    log(f"learned on crop, {k} iteration(s): {parity:.4f} of depths within 1% of the oracle "
-        f"running the same learned algorithm")
+    log(f"learned on the {w}x{h} crop, {k} iteration(s): {parity:.4f} of depths within 1% of the "
+        f"oracle running the same learned algorithm")
     ceng.close()
     pd.close()
     sd.close()
+
     # full-view timing at k (learned) and at the plain method's own count
     def timed(iters, learned):
         eng.set_num_iterations(iters)
@@ -223,7 +257,9 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, steps, warmup, tau,
         "speedup_vs_plain_at_equal_accuracy": wall_p / wall,
         "within_1pct_of_ground_truth": full_gt,
         "iterations_to_99pct_of_converged_oracle": need,
+        "iterations_to_plain_converged_ground_truth_accuracy": need_gt,
         "sweep_study": study,
+        "sweep_study_sample": where,
         "parity": {"within_1pct_of_fp64_oracle": parity,
                    "sample": f"{w}x{h} centred crop, {k} iteration(s), oracle given the same "
                              f"proposal planes"},
@@ -377,17 +413,47 @@ def batch_bench(args):
     return line
 
 
-def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads, fp16=False):
-    """Oracle (FP64, OpenMP) and the GPU engine on the same reference crop.
-    With fp16 the engine stores FP16 texels while the oracle reads the FP32
+def host_info(threads):
+    """The host the CPU baseline ran on: logical CPUs, the CPUs this process
+    may use, the CPU model and the oracle threads used."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "usable_cpus": usable, "cpu_model": model,
+            "threads": threads}
+
+
+def cpu_baseline_and_accuracy(sc, rp, sample, ctx, gpu_depth, threads, fp16=False):
+    """The FP64 oracle (OpenMP) on the timed view and parameters, from the
+    raw images (its own pre-blur, box sums, homographies, generator and
+    initial depth), against the engine's depth map of the timed run.
+    sample None: the whole view (gpu_depth is the timed run's depth map);
+    (w, h): the centred crop, run again on the GPU for the comparison.  With
+    fp16 the engine stores FP16 texels while the oracle reads the FP32
     images: the agreement is then the error FP16 storage adds."""
     from oracle import pmx_oracle as O
     from patchmatch_cuda_amd import capi, scene
 
     W, H = sc.spec.W, sc.spec.H
-    w, h = sample
-    x0, y0 = (W - w) // 2, (H - h) // 2
-    ref, srcs, cams, gt = scene.crop_reference(sc, x0, y0, w, h)
+    if sample is None:
+        w, h = W, H
+        ref, srcs, cams, gt = sc.ref, sc.srcs, sc.cams, sc.gt_depth
+        what = f"the whole {W}x{H} reference view"
+    else:
+        w, h = sample
+        x0, y0 = (W - w) // 2, (H - h) // 2
+        ref, srcs, cams, gt = scene.crop_reference(sc, x0, y0, w, h)
+        what = f"the centred {w}x{h} crop of the {W}x{H} reference view"
     O.set_threads(threads)
     f = lambda x: float(np.float32(x))  # noqa: E731
     t0 = time.perf_counter()
@@ -399,9 +465,9 @@ def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads, fp16=Fa
     st = O.rng_init_grid(w, h, rp.block_x, rp.block_y, rp.rng_seed)
     d0 = O.init_depth(st, f(rp.depth_near), f(rp.depth_far), rp.init_mode)
     d_o, _, _ = pb.run(d0, st, rp.num_samples, rp.num_iterations, record_ids=False)
-    t1 = time.perf_counter()
-    cpu_s = t1 - t0
-    log(f"cpu oracle on {w}x{h} crop: {cpu_s:.2f} s with {O.get_threads()} threads")
+    cpu_s = time.perf_counter() - t0
+    log(f"cpu oracle on {w}x{h}: {cpu_s:.2f} s with {O.get_threads()} threads")
+    hi = host_info(O.get_threads())
     out = {
         "cpu_baseline": {
             "value": (w * h) / cpu_s / 1e6,
@@ -409,12 +475,15 @@ def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads, fp16=Fa
             "cores": O.get_threads(),
             "kind": "port",
             "sample": (f"FP64 oracle, full pipeline (pre-blur, box sums, RNG, init, "
-                       f"{rp.num_iterations} iterations x 4 sweeps) on the centred {w}x{h} "
-                       f"crop of the reference view against all {len(srcs)} full-size "
-                       f"sources; {cpu_s:.2f} s wall"),
+                       f"{rp.num_iterations} iterations x 4 sweeps) on {what} against all "
+                       f"{len(srcs)} full-size sources, OpenMP over sweep columns"),
+            "wall_s": cpu_s,
+            **hi,
         }
     }
-    if do_accuracy:
+    if sample is None:
+        d_g = np.asarray(gpu_depth, np.float64)
+    else:
         eng = capi.Engine(ctx, rp, ref, list(srcs), cams)
         if fp16:
             eng.set_storage(capi.STORE_F16)
@@ -422,15 +491,16 @@ def cpu_baseline_and_accuracy(sc, rp, sample, ctx, do_accuracy, threads, fp16=Fa
         eng.run()
         d_g = eng.depth().astype(np.float64)
         eng.close()
-        agree = float(np.mean(np.abs(d_g - d_o) <= 0.01 * np.abs(d_o)))
-        out["accuracy"] = {
-            "within_1pct_of_fp64_oracle": agree,
-            "sample": f"{w}x{h} centred crop, same engine and parameters"
-                      + (", FP16 texels against the oracle on FP32 images" if fp16 else ""),
-            "gpu_within_1pct_of_ground_truth": float(np.mean(np.abs(d_g - gt) <= 0.01 * gt)),
-            "oracle_within_1pct_of_ground_truth": float(np.mean(np.abs(d_o - gt) <= 0.01 * gt)),
-        }
-        log(f"accuracy on crop: {agree:.4f} of depths within 1% of the oracle")
+    agree = float(np.mean(np.abs(d_g - d_o) <= 0.01 * np.abs(d_o)))
+    out["accuracy"] = {
+        "within_1pct_of_fp64_oracle": agree,
+        "sample": what + (" (the timed run's depth map)" if sample is None else "")
+                  + (", FP16 texels against the oracle on FP32 images" if fp16 else ""),
+        "pixels": w * h,
+        "gpu_within_1pct_of_ground_truth": float(np.mean(np.abs(d_g - gt) <= 0.01 * gt)),
+        "oracle_within_1pct_of_ground_truth": float(np.mean(np.abs(d_o - gt) <= 0.01 * gt)),
+    }
+    log(f"accuracy on {w}x{h}: {agree:.4f} of depths within 1% of the oracle")
     return out, d_o
 
 
@@ -440,10 +510,14 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2_realistic")
-    ap.add_argument("--cpu-sample", default="768x576",
-                    help="reference crop for the CPU oracle baseline and accuracy gate")
+    ap.add_argument("--cpu-sample", default="full",
+                    help="'full' (the whole timed view) or a centred crop WxH for the CPU "
+                         "oracle baseline and the accuracy gate")
+    ap.add_argument("--parity-crop", default="640x480",
+                    help="centred crop of the learned path's parity check against the oracle")
     ap.add_argument("--cpu-threads", type=int, default=0,
-                    help="oracle threads (default: min(16, cpu count))")
+                    help="oracle threads (default: min(16, usable CPUs): the CPU share of "
+                         "one GPU on the measurement host)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-waves", type=int, default=None,
                     help="wavefronts per sweep column (1, 2, 4; 0 = automatic; default: the "
@@ -522,7 +596,7 @@ def main():
             "sweep_waves": eng.sweep_waves(),
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "pipe": "VALU FP32 (NCC warp + bilinear + window sums; no matrix ops)",
             "kernel": "pmx_sweep_kernel",
             "achieved": achieved,
@@ -564,9 +638,10 @@ def main():
     weights = args.weights or proposal.DEFAULT_WEIGHTS
     d_oracle = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        extra, d_oracle = cpu_baseline_and_accuracy(sc, rp, parse_sample(args.cpu_sample), ctx,
-                                                    True, threads, args.fp16)
+        threads = args.cpu_threads or min(16, host_info(0)["usable_cpus"])
+        sample = None if args.cpu_sample == "full" else parse_sample(args.cpu_sample)
+        extra, d_oracle = cpu_baseline_and_accuracy(sc, rp, sample, ctx, plain_depth, threads,
+                                                    args.fp16)
         line.update(extra)
     if args.no_learned:
         pass
@@ -575,10 +650,15 @@ def main():
     elif d_oracle is None:
         line["learned"] = {"skipped": "needs the oracle crop (--no-cpu-baseline given)"}
     else:
-        lr = learned_leg(ctx, eng, sc, rp, weights, d_oracle, parse_sample(args.cpu_sample),
-                         args.steps, args.warmup, args.tau, bool(args.proposal_seeding))
-        lr["speedup_vs_plain"] = lr["value"] / value
-        lr["sweep_reduction_vs_plain"] = (4 * rp.num_iterations) / lr["sweeps"]
+        lr = learned_leg(ctx, eng, sc, rp, weights, d_oracle,
+                         None if args.cpu_sample == "full" else parse_sample(args.cpu_sample),
+                         parse_sample(args.parity_crop), args.steps, args.warmup, args.tau,
+                         bool(args.proposal_seeding))
+        # both at the iteration count each needs to reach the accuracy bar
+        need = lr["iterations_to_99pct_of_converged_oracle"]
+        lr["speedup_vs_plain"] = lr["speedup_vs_plain_at_equal_accuracy"]
+        lr["sweep_reduction_vs_plain"] = ((need["plain"] or rp.num_iterations) /
+                                          (need["learned"] or rp.num_iterations))
         line["learned"] = lr
     eng.close()
     ctx.close()
