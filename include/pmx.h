@@ -150,6 +150,16 @@ int pmx_box_sums(pmx_ctx* ctx, const float* img, int img_pitch_bytes, int width,
  * (plane k starts at row k*height), clamp-to-edge. */
 int pmx_blur(pmx_ctx* ctx, float* data, int pitch_bytes, int width, int height,
              int depth, float sigma, int blur_mode);
+/* pmx_blur without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The row pass goes to a
+ * library-owned scratch of 2^23 floats per GPU, so width * height must be
+ * at most 8388608 (PMX_ERR_ARG otherwise).  Stream-ordered on the context's
+ * stream (no synchronisation; pmx_ctx_sync waits).  Every call on one GPU
+ * shares that scratch: calls that overlap on one GPU (other contexts or
+ * threads) race. */
+int pmx_blur_noalloc(pmx_ctx* ctx, float* data, int pitch_bytes, int width, int height,
+                     int depth, float sigma, int blur_mode);
 /* Image::init_relative (src/image.cpp:4-13) for num_src views, composed in
  * FP64 on the GPU.  H_out: host, num_src * 18 doubles (H1 row-major, H2). */
 int pmx_compose_homographies(pmx_ctx* ctx, const pmx_camera* ref,

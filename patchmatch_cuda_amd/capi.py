@@ -92,6 +92,7 @@ SIGNATURES = {
     "pmx_rng_fill_depth": [_vp, _vp, _i, _vp, _i, _i, _i, _f, _f, _i],
     "pmx_box_sums": [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i],
     "pmx_blur": [_vp, _vp, _i, _i, _i, _i, _f, _i],
+    "pmx_blur_noalloc": [_vp, _vp, _i, _i, _i, _i, _f, _i],
     "pmx_compose_homographies": [_vp, ctypes.POINTER(Camera), ctypes.POINTER(Camera), _i, _f, _f,
                                  _vp],
     "pmx_validate_params": [ctypes.POINTER(Params), _i],

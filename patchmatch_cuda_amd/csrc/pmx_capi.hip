@@ -16,10 +16,20 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "pmx_internal.h"
+
+/* Scratch of pmx_blur_noalloc: the row pass of the separable filter writes
+ * one width x height plane here, the column pass reads it back.  File-scope
+ * device memory, one copy per GPU for the life of the process; host code
+ * never passes this name to a kernel, only the address that
+ * hipGetSymbolAddress returns for the current device (kept per device by
+ * pmx_ctx_create). */
+static constexpr size_t kBlurScratchFloats = (size_t)1 << 23; /* 32 MiB: a 4096 x 2048 plane */
+static __device__ __align__(256) float scratch[kBlurScratchFloats];
 
 namespace {
 
@@ -65,6 +75,27 @@ int dev_alloc(pmx_ctx* ctx, void** p, size_t bytes) {
 
 void dev_free(void* p) {
     if (p) (void)hipFree(p);
+}
+
+/* Address of `scratch` on each device, set once per device by the first
+ * context created on it (with that device current). */
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[kMaxDevices] = {};
+
+int scratch_setup(pmx_ctx* ctx) {
+    if (ctx->device >= kMaxDevices)
+        return fail(ctx, PMX_ERR_ARG, "device %d beyond the %d devices the library supports",
+                    ctx->device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[ctx->device]) return PMX_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(ctx, PMX_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s",
+                    ctx->device, hipGetErrorString(e));
+    g_scratch[ctx->device] = (float*)p;
+    return PMX_OK;
 }
 
 inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
@@ -357,6 +388,13 @@ int pmx_ctx_create(int device, pmx_ctx** out) {
         delete ctx;
         return fail(nullptr, PMX_ERR_HIP, "context creation failed: %s", hipGetErrorString(e));
     }
+    const int rc = scratch_setup(ctx);
+    if (rc != PMX_OK) {
+        const std::string msg = ctx->err;
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(nullptr, rc, "%s", msg.c_str());
+    }
     *out = ctx;
     return PMX_OK;
 }
@@ -508,6 +546,28 @@ int pmx_blur(pmx_ctx* ctx, float* data, int pitch, int width, int height, int de
     dev_free(tmp);
     HIPCHK(ctx, e);
     HIPCHK(ctx, e2);
+    return PMX_OK;
+}
+
+int pmx_blur_noalloc(pmx_ctx* ctx, float* data, int pitch, int width, int height, int depth,
+                     float sigma, int mode) {
+    if (!ctx || !data) return fail(ctx, PMX_ERR_ARG, "NULL argument to pmx_blur_noalloc");
+    if (mode == PMX_BLUR_NONE) return PMX_OK;
+    if (mode != PMX_BLUR_GAUSS && mode != PMX_BLUR_BOX)
+        return fail(ctx, PMX_ERR_ARG, "unknown blur mode %d", mode);
+    if (!(sigma > 0.0f) || width <= 0 || height <= 0 || depth <= 0 || pitch % 4 ||
+        pitch < width * 4)
+        return fail(ctx, PMX_ERR_ARG, "invalid blur arguments");
+    if ((size_t)width * (size_t)height > kBlurScratchFloats)
+        return fail(ctx, PMX_ERR_ARG,
+                    "pmx_blur_noalloc: a %d x %d plane exceeds its scratch of %zu floats "
+                    "(pmx_blur has no such limit)", width, height, kBlurScratchFloats);
+    float* tmp = ctx->device < kMaxDevices ? g_scratch[ctx->device] : nullptr;
+    if (!tmp) return fail(ctx, PMX_ERR_STATE, "no scratch address for device %d", ctx->device);
+    float taps[45];
+    const int n = blur_taps(sigma, mode, taps);
+    HIPCHK(ctx, pmx_launch_blur(ctx->stream, data, pitch / 4, width, height, depth,
+                                (long)(pitch / 4) * height, taps, n, tmp));
     return PMX_OK;
 }
 
