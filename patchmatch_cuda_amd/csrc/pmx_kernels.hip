@@ -481,8 +481,12 @@ __device__ __forceinline__ TapW tap_fetch(const SB& sb, const Hyp& q, float sx, 
     sy = __builtin_amdgcn_fmed3f(sy, q.loy, q.hiy);
     const int ix = floor_i(sx), iy = floor_i(sy);
     /* |iy| < 2^17 and the pitch < 2^17: a 24-bit multiply is exact */
-    const uint32_t off =
-        (uint32_t)(__mul24(iy, (int)sb.pitch_bytes) + (int)q.base) + ((uint32_t)ix << SB::kShift);
+    /* iy * pitch + (ix << shift) + base as a shift-add and one signed
+     * 24-bit multiply-add (the compiler otherwise re-associates it into
+     * three instructions) */
+    const uint32_t colb = ((uint32_t)ix << SB::kShift) + q.base;
+    uint32_t off;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(off) : "v"(iy), "v"(sb.pitch_bytes), "v"(colb));
     if constexpr (SB::kShift == 1) {
         t0 = __builtin_amdgcn_raw_buffer_load_b32(sb.rsrc, off, 0, 0);
         t1 = __builtin_amdgcn_raw_buffer_load_b32(sb.rsrc, off, sb.pitch_bytes, 0);
@@ -1004,10 +1008,21 @@ __device__ __forceinline__ double sentinel_emission(float cost, bool is_view, in
     return e;
 }
 
+/* exp(t) for t <= 0 in FP64 range: t log2(e) split into an integer n and a
+ * fraction f in [-1/2, 1/2]; 2^f by the hardware FP32 exp2 (1 ulp), scaled
+ * by 2^n in FP64.  Relative error ~1.5e-7 down to FP64's denormals, where
+ * results round to 0 as an FP64 exp does. */
+__device__ __forceinline__ double exp_neg(double t) {
+    const double y = fmax(t * 1.4426950408889634, -2200.0);
+    const double n = rint(y);
+    const float f = (float)(y - n);
+    return ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
+}
+
 /* computeMessageBackward (src/patchMatch.cu:669-690) */
 __device__ __forceinline__ double msg_backward(double m, double e, const MsgConst& mc) {
     const double fix = (double)kFix, chg = (double)kChange;
-    const double em = exp(-e * e * mc.inv2a2);
+    const double em = exp_neg(-e * e * mc.inv2a2);
     const double z0 = m * em * fix + (1.0 - m) * mc.up * chg;
     const double z1 = m * em * chg + (1.0 - m) * mc.up * fix;
     return z0 / (z0 + z1); /* a true divide: z0 + z1 may be denormal */
@@ -1016,7 +1031,7 @@ __device__ __forceinline__ double msg_backward(double m, double e, const MsgCons
 /* computeMessageForward (src/patchMatch.cu:692-715) */
 __device__ __forceinline__ double msg_forward(double m, double e, const MsgConst& mc) {
     const double fix = (double)kFix, chg = (double)kChange;
-    const double em = exp(-e * e * mc.inv2a2);
+    const double em = exp_neg(-e * e * mc.inv2a2);
     const double z0 = (m * fix + (1.0 - m) * chg) * em;
     const double z1 = (m * chg + (1.0 - m) * fix) * mc.up;
     return z0 / (z0 + z1);
@@ -1243,15 +1258,27 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         const int srank = __popcll(sel & ((1ull << lane) - 1ull));
         /* a learned proposal is a fourth candidate on the same views */
         const bool has_prop = d_prop > 0.0f;
-        const int per = has_prop ? 3 : 2;
+        /* The former depth equal to the current one: its NCC on a view is
+         * the stored cost of that view (the same function of the same
+         * pixel, depth and images), so former and current tie exactly and
+         * findMinCost keeps the current depth unless the random (or
+         * proposal) candidate is at least as good.  Only the left and top
+         * border columns, where the reference's tile-index shift makes the
+         * NCC depend on the sweep frame the stored cost came from, evaluate
+         * the former depth anyway. */
+        const bool same = d_former == d_cur && x >= a.hw && y >= a.hw;
+        const int per = (same ? 1 : 2) + (has_prop ? 1 : 0);
         if (in_sel) {
-            tview[per * srank] = lane;
-            tview[per * srank + 1] = lane;
-            tdep[per * srank] = d_former;
-            tdep[per * srank + 1] = d_rand;
+            int o = per * srank;
+            if (!same) {
+                tview[o] = lane;
+                tdep[o++] = d_former;
+            }
+            tview[o] = lane;
+            tdep[o++] = d_rand;
             if (has_prop) {
-                tview[per * srank + 2] = lane;
-                tdep[per * srank + 2] = d_prop;
+                tview[o] = lane;
+                tdep[o] = d_prop;
             }
         }
         if (NW == 1) __syncthreads();
@@ -1261,9 +1288,10 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         run_phase(nt, win, tres, pc);
         float nf = 0.0f, nr = 0.0f, np = 0.0f;
         if (in_sel) {
-            nf = tres[per * srank];
-            nr = tres[per * srank + 1];
-            if (has_prop) np = tres[per * srank + 2];
+            int o = per * srank;
+            nf = same ? cst : tres[o++];
+            nr = tres[o++];
+            if (has_prop) np = tres[o];
         }
         float c0 = 0.0f, c1s = 0.0f, c2 = 0.0f, c3 = 0.0f;
         for (int j = 0; j < a.S; j++) {
