@@ -76,7 +76,27 @@ __device__ __forceinline__ double row_scan16(double v) {
     return v;
 }
 
+/* Workgroup barrier that orders LDS only: a wave's LDS writes are complete
+ * before any wave passes it, while its global loads stay in flight
+ * (__syncthreads also waits for every outstanding global load, which would
+ * expose the latency of the next row's prefetch at each barrier). */
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 /* Value of lane `src` (wave-uniform) in every lane. */
+__device__ __forceinline__ float lane_f(float v, int src);
+
+/* Sum over lanes 0..n-1 (the other lanes hold 0), the same in every lane. */
+__device__ __forceinline__ float wave_sum(float v, int n) {
+    const float r = row_sum16(v);
+    float s = lane_f(r, 0);
+    if (n > 16) s = (s + lane_f(r, 16)) + (lane_f(r, 32) + lane_f(r, 48));
+    return s;
+}
+
 __device__ __forceinline__ float lane_f(float v, int src) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
 }
@@ -1019,13 +1039,30 @@ __device__ __forceinline__ double exp_neg(double t) {
     return ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
 }
 
+/* n / d for a probability, 0 <= n <= d: the hardware reciprocal refined by
+ * two Newton steps and one correction of the quotient (within an ulp of
+ * the quotient) where d is a normal number, a true divide where it is not:
+ * with messages near 0 or 1 both terms can be denormal, where a reciprocal
+ * overflows.  The wave takes the slow path only when one of its lanes needs
+ * it (every lane must call this). */
+__device__ __forceinline__ double div_prob(double n, double d) {
+    if (__ballot(!(d >= 0x1p-1000)) == 0) {
+        double r = __builtin_amdgcn_rcp(d);
+        r = fma(r, fma(-d, r, 1.0), r);
+        r = fma(r, fma(-d, r, 1.0), r);
+        const double q = n * r;
+        return fma(r, fma(-d, q, n), q);
+    }
+    return n / d;
+}
+
 /* computeMessageBackward (src/patchMatch.cu:669-690) */
 __device__ __forceinline__ double msg_backward(double m, double e, const MsgConst& mc) {
     const double fix = (double)kFix, chg = (double)kChange;
     const double em = exp_neg(-e * e * mc.inv2a2);
     const double z0 = m * em * fix + (1.0 - m) * mc.up * chg;
     const double z1 = m * em * chg + (1.0 - m) * mc.up * fix;
-    return z0 / (z0 + z1); /* a true divide: z0 + z1 may be denormal */
+    return div_prob(z0, z0 + z1);
 }
 
 /* computeMessageForward (src/patchMatch.cu:692-715) */
@@ -1034,7 +1071,7 @@ __device__ __forceinline__ double msg_forward(double m, double e, const MsgConst
     const double em = exp_neg(-e * e * mc.inv2a2);
     const double z0 = (m * fix + (1.0 - m) * chg) * em;
     const double z1 = (m * chg + (1.0 - m) * fix) * mc.up;
-    return z0 / (z0 + z1);
+    return div_prob(z0, z0 + z1);
 }
 
 /* ---------------------------------------------------------------- sweep */
@@ -1088,7 +1125,6 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     float* tres = winbuf + 2 * WW;                       /* task results: kTasks, then 64 */
     int* tview = (int*)(tres + kTasks + 64);             /* task views, kTasks */
     float* tdep = (float*)(tview + kTasks);              /* task depths, kTasks */
-    int* vj = (int*)(tdep + kTasks);                     /* sampled views, S */
 
     constexpr int NG = kLanes / G;
     constexpr int NGT = NG * NW;
@@ -1188,7 +1224,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             float4* pp = part + (pb & 1) * 64 * NW;
             const int kk = eval_pass<G, WN, NW, NB>(a, Hs, sb, win, wave, lane, ntp, tview + t0,
                                                 tdep + t0, tout + t0, pp, pc);
-            __syncthreads();
+            lds_barrier();
             if (kk > NG) combine_pass<NW>(lane, ntp, kk / NG, pp, tout + t0, pc);
             pb++;
         }
@@ -1196,7 +1232,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
 
     for (int k = 1; k < Hs_rows; k++, r += step) {
         float* win = winbuf + buf * WW;
-        __syncthreads();
+        lds_barrier();
         const float cst = pf_cost, d_cur = pf_depth, spread = pf_spread;
         const double beta = pf_beta;
         float d_prop = pf_prop;
@@ -1218,13 +1254,12 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         const double m_prev = m;
         double e = sentinel_emission(cst, is_view, lane, rng);
         const double mf = msg_forward(m, e, mc);
-        double q = 0.0;
-        if (is_view) {
+        double q;
+        {
             const double z0 = mf * beta;
             const double z1 = (1.0 - mf) * (1.0 - beta);
-            /* a true divide: with messages near 0 or 1 both terms can be
-             * denormal, where a reciprocal overflows */
-            q = z0 / (z0 + z1);
+            q = div_prob(z0, z0 + z1);
+            q = is_view ? q : 0.0;
         }
         /* inclusive prefix sum over lanes 0..V-1, the normalised CDF of
          * src/patchMatch.cu:917-923 */
@@ -1239,19 +1274,40 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             }
         }
         const double total = lane_d(cdf, V - 1);
-        cdf = (lane == V - 1) ? 1.0 : cdf / total;
+        {
+            const double nc = div_prob(cdf, total);
+            cdf = (lane == V - 1) ? 1.0 : nc;
+        }
 
-        /* draws: random depth, then S views (src/patchMatch.cu:937-961) */
+        /* draws: random depth, then S views (src/patchMatch.cu:937-961).
+         * A view draw takes the first view whose CDF entry is >= u (none:
+         * a NaN CDF, the last view).  floor(64 / V) draws are resolved by one
+         * ballot: lane d V + i compares draw d with view i's CDF entry. */
         const float d_rand = xw_uniform(rng) * (a.far_d - a.near_d) + a.near_d;
         uint64_t sel = 0;
-        for (int j = 0; j < a.S; j++) {
-            const double u = (double)xw_uniform(rng);
-            const uint64_t hit = __ballot(is_view && u <= cdf);
-            const int id = hit ? (int)__builtin_ctzll(hit) : V - 1;
-            sel |= 1ull << id;
-            if (lane == 0) vj[j] = id;
-            if (lead && a.ids && lane == 0)
-                a.ids[((size_t)j * a.H + y) * a.ids_pitch + x] = (uint8_t)id;
+        int cnt = 0; /* draws that picked this lane's view */
+        {
+            const int per_b = kLanes / V;
+            const int ld = lane / V, lv = lane - ld * V;
+            const double cv = __shfl(cdf, lv, kLanes);
+            const uint64_t vmask = (V >= 64) ? ~0ull : ((1ull << V) - 1ull);
+            for (int j0 = 0; j0 < a.S; j0 += per_b) {
+                const int nd = min(per_b, a.S - j0);
+                float uj = 2.0f;
+                for (int t = 0; t < nd; t++) {
+                    const float u = xw_uniform(rng);
+                    uj = (ld == t) ? u : uj;
+                }
+                const uint64_t hit = __ballot(ld < nd && (double)uj <= cv);
+                for (int t = 0; t < nd; t++) {
+                    const uint64_t f = (hit >> (t * V)) & vmask;
+                    const int id = f ? (int)__builtin_ctzll(f) : V - 1;
+                    sel |= 1ull << id;
+                    cnt += (lane == id) ? 1 : 0;
+                    if (lead && a.ids && lane == 0)
+                        a.ids[((size_t)(j0 + t) * a.H + y) * a.ids_pitch + x] = (uint8_t)id;
+                }
+            }
         }
         const int U = __popcll(sel);
         const bool in_sel = is_view && ((sel >> lane) & 1ull);
@@ -1281,7 +1337,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                 tdep[o] = d_prop;
             }
         }
-        if (NW == 1) __syncthreads();
+        if (NW == 1) lds_barrier();
 
         /* phase B: former and random depth (and the proposal) on the sampled views */
         const int nt = per * U;
@@ -1293,13 +1349,15 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             nr = tres[o++];
             if (has_prop) np = tres[o];
         }
-        float c0 = 0.0f, c1s = 0.0f, c2 = 0.0f, c3 = 0.0f;
-        for (int j = 0; j < a.S; j++) {
-            const int id = __builtin_amdgcn_readfirstlane(vj[j]);
-            c0 += lane_f(nf, id);
-            c1s += lane_f(cst, id);
-            c2 += lane_f(nr, id);
-            if (has_prop) c3 += lane_f(np, id);
+        /* candidate costs summed over the S draws: each view's cost times
+         * the number of draws that picked it, summed over the views */
+        float c0, c1s, c2, c3 = 0.0f;
+        {
+            const float k = is_view ? (float)cnt : 0.0f;
+            c0 = wave_sum(is_view ? k * nf : 0.0f, V);
+            c1s = wave_sum(is_view ? k * cst : 0.0f, V);
+            c2 = wave_sum(is_view ? k * nr : 0.0f, V);
+            if (has_prop) c3 = wave_sum(is_view ? k * np : 0.0f, V);
         }
         const float inv_s = 1.0f / (float)a.S;
         c0 *= inv_s;
@@ -1331,7 +1389,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                 tview[rrank] = lane;
                 tdep[rrank] = best;
             }
-            if (NW == 1) __syncthreads();
+            if (NW == 1) lds_barrier();
             run_phase(nrest, win, tres + kTasks, pc);
             if (in_sel) cnew = (idx == 0) ? nf : (idx == 2 ? nr : np);
             if (in_rest) cnew = tres[kTasks + rrank];
