@@ -224,6 +224,15 @@ int pmx_engine_get_dims(pmx_engine* eng, int* width, int* height, int* num_src,
 int pmx_engine_export(pmx_engine* eng, float** depth, int* depth_pitch,
                       float** costs, int* cost_pitch, uint8_t** used_ids,
                       int* ids_pitch);
+/* PatchMatch::_refImage (src/patchMatch.h:41, Array2d_refImg filled by the
+ * constructor, src/patchMatch.cu:168-175) and _psngState (:49, seeded at
+ * :164 and advanced by every sweep): device views of the engine's reference
+ * image I (pre-blurred), its window sums of I and I^2 (W x H floats at
+ * ref_pitch bytes) and its per-pixel generator states (W x H at rng_pitch
+ * bytes).  Valid from pmx_engine_prepare until the engine is destroyed. */
+int pmx_engine_export_state(pmx_engine* eng, float** ref, float** sum_I,
+                            float** sum_II, int* ref_pitch, pmx_rng_state** rng,
+                            int* rng_pitch);
 int pmx_engine_set_timing(pmx_engine* eng, int enabled);
 /* Launch shape of the sweep kernel: wavefronts per sweep column (1, 2 or
  * 4), or 0 = automatic, the default (two waves per column where every

@@ -100,6 +100,12 @@ def build_drop_in():
         _atomic(CLI, lambda o: [CXX, "-O2", "-std=c++17", "-Wall", *inc, "-o", o, cli,
                                 f"-L{HERE}", "-lpmx_drop_in", "-lpmx", f"-Wl,-rpath,{HERE}",
                                 "-Wl,-rpath,$ORIGIN"])
+    # the C++ test program of the helper classes (run by tests/test_dropin_helpers_gpu.py)
+    prog = os.path.join(ROOT, "tests", "dropin_helpers.cpp")
+    if os.path.exists(prog):
+        _atomic(prog[:-4], lambda o: [CXX, "-O2", "-std=c++17", "-Wall", *inc, "-o", o, prog,
+                                      f"-L{HERE}", "-lpmx_drop_in", "-lpmx",
+                                      f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../patchmatch_cuda_amd"])
 
 
 def build_oracle():

@@ -118,6 +118,9 @@ SIGNATURES = {
     "pmx_engine_get_dims": [_vp] + [ctypes.POINTER(_i)] * 5,
     "pmx_engine_export": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i), ctypes.POINTER(_vp),
                           ctypes.POINTER(_i), ctypes.POINTER(_vp), ctypes.POINTER(_i)],
+    "pmx_engine_export_state": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                ctypes.POINTER(_vp), ctypes.POINTER(_i), ctypes.POINTER(_vp),
+                                ctypes.POINTER(_i)],
     "pmx_engine_set_timing": [_vp, _i],
     "pmx_engine_set_sweep_waves": [_vp, _i],
     "pmx_engine_get_sweep_waves": [_vp, ctypes.POINTER(_i)],

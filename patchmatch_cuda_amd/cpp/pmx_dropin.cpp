@@ -285,6 +285,11 @@ Array2D_psng::Array2D_psng(int width, int height, int blockDim_x, int blockDim_y
     randStateGen();
 }
 
+Array2D_psng::Array2D_psng(pmx_rng_state* dev, int pitch, int width, int height, int blockDim_x,
+                           int blockDim_y)
+    : Array2D_wrapper<pmx_rng_state>(dev, pitch, width, height, 1, blockDim_x, blockDim_y),
+      _seed(0) {}
+
 void Array2D_psng::randStateGen() {
     /* src/array2d_psng.cu:20-31 */
     PMX_SAFE_CALL(pmx_rng_init(pmx_dropin_context(), _array2D, _pitchData, _width, _height,
@@ -354,6 +359,11 @@ Array2d_refImg::Array2d_refImg(int refWidth, int refHeight, int blockDim_x, int 
     _tempArray = new CudaArray_wrapper(refWidth, refHeight, 1);
     _tempArray_float = new CudaArray_wrapper(refWidth, refHeight, 1);
 }
+
+Array2d_refImg::Array2d_refImg(Array2D_wrapper<float>* I, Array2D_wrapper<float>* sum_I,
+                               Array2D_wrapper<float>* sum_II, int refWidth, int refHeight)
+    : _refImage_sum_II(sum_II), _refImage_sum_I(sum_I), _refImageData(I), _refWidth(refWidth),
+      _refHeight(refHeight), _tempArray(nullptr), _tempArray_float(nullptr) {}
 
 Array2d_refImg::~Array2d_refImg() {
     delete _refImage_sum_I;
@@ -444,7 +454,8 @@ PatchMatch::PatchMatch(std::vector<Image>& allImage, float nearRange, float farR
                        float orientationX, float orientationZ)
     : _nearRange(nearRange), _farRange(farRange), _halfWindowSize(halfWindowSize),
       _blockDim_x(blockDim_x), _blockDim_y(blockDim_y), _refImageId(refImageId),
-      _depthMap(nullptr), _matchCost(nullptr), _usedImgsID(nullptr), _maxWidth(0), _maxHeight(0),
+      _depthMap(nullptr), _matchCost(nullptr), _usedImgsID(nullptr), _refImage(nullptr),
+      _psngState(nullptr), _maxWidth(0), _maxHeight(0),
       _transformHH(nullptr), _numOfSamples(numOfSamples), _SPMAlpha(SPMAlpha),
       _gaussianSigma(gaussianSigma), _numOfIterations(numOfIterations),
       _orientationX(orientationX), _orientationZ(orientationZ), _engine(nullptr) {
@@ -527,9 +538,21 @@ PatchMatch::PatchMatch(std::vector<Image>& allImage, float nearRange, float farR
     for (size_t i = 0; i < H.size(); i++) _transformHH[i] = (float)H[i];
     computeCUDAConfig(_refWidth, _refHeight, blockDim_x, blockDim_y);
     bindOutputs();
+    /* _refImage and _psngState (src/patchMatch.cu:164-179) as views */
+    float *I = nullptr, *sI = nullptr, *sII = nullptr;
+    pmx_rng_state* st = nullptr;
+    int rp = 0, sp = 0;
+    PMX_SAFE_CALL(pmx_engine_export_state(_engine, &I, &sI, &sII, &rp, &st, &sp));
+    auto view = [&](float* d) {
+        return new Array2D_wrapper<float>(d, rp, _refWidth, _refHeight, 1, _blockDim_x, _blockDim_y);
+    };
+    _refImage = new Array2d_refImg(view(I), view(sI), view(sII), _refWidth, _refHeight);
+    _psngState = new Array2D_psng(st, sp, _refWidth, _refHeight, _blockDim_x, _blockDim_y);
 }
 
 PatchMatch::~PatchMatch() {
+    delete _refImage;
+    delete _psngState;
     delete _depthMap;
     delete _matchCost;
     delete _usedImgsID;

@@ -126,7 +126,11 @@ public:
     Array2D_psng(int width, int height, int blockDim_x, int blockDim_y, uint32_t seed = 7u);
 
 private:
+    /* a non-owning view of an engine's generator states (PatchMatch::_psngState) */
+    Array2D_psng(pmx_rng_state* dev, int pitch, int width, int height, int blockDim_x,
+                 int blockDim_y);
     uint32_t _seed;
+    friend class PatchMatch;
 };
 
 /* src/cudaArray_wrapper.h:6-25: a layered width x height x depth image.
@@ -173,6 +177,13 @@ public:
      * sums of I and I^2 with zero border; hw outside 1..22 prints the
      * reference's warning and exits with status 0 (:96). */
     void filterImage(int halfWindowSize);
+
+private:
+    /* views of an engine's reference image and window sums
+     * (PatchMatch::_refImage); takes the three wrapper objects */
+    Array2d_refImg(Array2D_wrapper<float>* I, Array2D_wrapper<float>* sum_I,
+                   Array2D_wrapper<float>* sum_II, int refWidth, int refHeight);
+    friend class PatchMatch;
 };
 
 /* src/cudaTranspose.h:52-71 (4-byte element types) */
@@ -221,6 +232,11 @@ public:
     Array2D_wrapper<float>* _depthMap;
     Array2D_wrapper<float>* _matchCost;
     Array2D_wrapper<uchar>* _usedImgsID;
+    /* src/patchMatch.h:41, :49: the reference image (pre-blurred) with its
+     * window sums, and the per-pixel generator states; views of the
+     * engine's device memory, valid until destruction */
+    Array2d_refImg* _refImage;
+    Array2D_psng* _psngState;
 
     int _numOfTargetImages;
     int _maxWidth;

@@ -1279,6 +1279,19 @@ int pmx_engine_export(pmx_engine* e, float** depth, int* depth_pitch, float** co
     return PMX_OK;
 }
 
+int pmx_engine_export_state(pmx_engine* e, float** ref, float** sum_I, float** sum_II,
+                            int* ref_pitch, pmx_rng_state** rng, int* rng_pitch) {
+    PMXCHK(check_engine(e));
+    PMXCHK(require_prepared(e));
+    if (ref) *ref = e->ref;
+    if (sum_I) *sum_I = e->sum_I;
+    if (sum_II) *sum_II = e->sum_II;
+    if (ref_pitch) *ref_pitch = e->ref_pitch * 4;
+    if (rng) *rng = e->rng;
+    if (rng_pitch) *rng_pitch = e->rng_pitch;
+    return PMX_OK;
+}
+
 int pmx_engine_set_timing(pmx_engine* e, int enabled) {
     PMXCHK(check_engine(e));
     e->timing = enabled != 0;
