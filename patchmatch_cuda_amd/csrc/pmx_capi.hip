@@ -814,7 +814,7 @@ int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format, 
     ALLOC(e->cost_rm, sizeof(float) * (size_t)e->cost_rm_ld * H);
     ALLOC(e->cost_cm, sizeof(float) * (size_t)e->cost_cm_ld * W);
     /* messages; also the scratch plane of the pre-blur */
-    ALLOC(e->spm, std::max(sizeof(double) * (size_t)W * H * V, sizeof(float) * (size_t)maxW * maxH));
+    ALLOC(e->spm, std::max(sizeof(float) * (size_t)W * H * V, sizeof(float) * (size_t)maxW * maxH));
     HIPCHK_E(hipMemsetAsync(e->cost_rm, 0, sizeof(float) * (size_t)e->cost_rm_ld * H, s));
     e->rng_pitch = (int)(sizeof(pmx_rng_state) * (size_t)W);
     ALLOC(e->rng, (size_t)e->rng_pitch * H);
@@ -861,7 +861,7 @@ int pmx_engine_footprint(const pmx_params* params, int ref_width, int ref_height
     t += g(sizeof(float) * (size_t)round_up(ref_height, 64) * W);     /* depth_cm */
     t += g(sizeof(float) * (size_t)round_up(ref_width * num_src, 64) * H);
     t += g(sizeof(float) * (size_t)round_up(ref_height * num_src, 64) * W);
-    t += g(std::max(sizeof(double) * W * H * V, sizeof(float) * (size_t)maxW * maxH));
+    t += g(std::max(sizeof(float) * W * H * V, sizeof(float) * (size_t)maxW * maxH));
     t += g(sizeof(pmx_rng_state) * W * H);
     t += g((size_t)round_up(ref_width, 64) * H * (size_t)params->num_samples);
     t += g(sizeof(float) * ref_pitch * H * V);                        /* out_cost */

@@ -56,7 +56,7 @@ struct PmxSweepArgs {
     int depth_ld;
     float* cost;           /* state: column c at cost + c*cost_ld, [row][view] */
     int cost_ld;
-    double* spm;           /* backward messages (FP64): column c at spm + c*spm_ld, [row][view] */
+    float* spm;            /* backward messages as log-odds: column c at spm + c*spm_ld, [row][view] */
     int spm_ld;
     pmx_rng_state* rng;    /* W x H */
     int rng_pitch;         /* bytes */
@@ -121,7 +121,7 @@ struct pmx_engine {
     float* depth_cm = nullptr;  int depth_cm_ld = 0;   /* column-major */
     float* cost_rm = nullptr;   int cost_rm_ld = 0;
     float* cost_cm = nullptr;   int cost_cm_ld = 0;
-    double* spm = nullptr;      int spm_ld = 0;
+    float* spm = nullptr;       int spm_ld = 0;
     int layout_cm = 0;          /* 1: state currently in the column-major buffers */
 
     pmx_rng_state* rng = nullptr;

@@ -48,14 +48,6 @@ __device__ __forceinline__ float dpp_f(float v) {
         float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
 }
 
-/* Lanes whose source falls outside the row read 0. */
-template <int CTRL>
-__device__ __forceinline__ double dpp_d(double v) {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
-}
 
 /* Sum over each 16-lane row, the same value in every lane of the row
  * (each step adds a lane pair symmetrically, so all lanes agree bitwise). */
@@ -67,14 +59,24 @@ __device__ __forceinline__ float row_sum16(float v) {
     return v;
 }
 
-/* Inclusive prefix sum over each 16-lane row. */
-__device__ __forceinline__ double row_scan16(double v) {
-    v += dpp_d<kDppRowShr + 1>(v);
-    v += dpp_d<kDppRowShr + 2>(v);
-    v += dpp_d<kDppRowShr + 4>(v);
-    v += dpp_d<kDppRowShr + 8>(v);
+/* Inclusive prefix sum over each 16-lane row (FP32). */
+__device__ __forceinline__ float row_scan16f(float v) {
+    v += dpp_f<kDppRowShr + 1>(v);
+    v += dpp_f<kDppRowShr + 2>(v);
+    v += dpp_f<kDppRowShr + 4>(v);
+    v += dpp_f<kDppRowShr + 8>(v);
     return v;
 }
+
+/* Maximum over each 16-lane row, the same in every lane of the row. */
+__device__ __forceinline__ float row_max16(float v) {
+    v = fmaxf(v, dpp_f<kDppQuadXor1>(v));
+    v = fmaxf(v, dpp_f<kDppQuadXor2>(v));
+    v = fmaxf(v, dpp_f<kDppRowHalfMirror>(v));
+    v = fmaxf(v, dpp_f<kDppRowMirror>(v));
+    return v;
+}
+
 
 /* Workgroup barrier that orders LDS only: a wave's LDS writes are complete
  * before any wave passes it, while its global loads stay in flight
@@ -101,12 +103,6 @@ __device__ __forceinline__ float lane_f(float v, int src) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
 }
 
-__device__ __forceinline__ double lane_d(double v, int src) {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
-}
 
 /* ------------------------------------------------------------------ RNG */
 /* Marsaglia xorwow (J. Stat. Softw. 8(14), 2003); seeding by SplitMix64
@@ -993,85 +989,61 @@ __global__ void seed_depth_kernel(const float* __restrict__ prop, int prop_ld,
 }
 
 /* ----------------------------------------------------------- messages */
-/* The view-selection HMM runs in FP64: the reference evaluates the emission
- * exp(-0.5 c^2 / alpha^2) in double (src/patchMatch.cu:683, :708), and the
- * costs that the left/top-border tile-index quirk produces (|c| up to ~100)
- * drive emissions far below the FP32 range, where FP32 messages would
- * collapse to exactly 0.  Constants keep their FP32 literal values. */
+/* The two-state view-selection HMM (src/patchMatch.cu:669-715) carried as
+ * log-odds l = log(m / (1 - m)) of "view visible", in FP32.  With the
+ * emission e(c) = exp(-c^2 / 2 alpha^2) of a visible view and the uniform
+ * level u = e(0.83), the reference's updates are
+ *     forward   m' = (m fix + (1-m) chg) e / [(m fix + (1-m) chg) e + (m chg + (1-m) fix) u]
+ *     backward  m' = (m e fix + (1-m) u chg) / [(m e fix + (1-m) u chg) + (m e chg + (1-m) u fix)]
+ * which in log-odds are  l' = G(l) + E  and  l' = G(l + E),  with the
+ * emission's log ratio E = log(e / u) = (0.83^2 - c^2) / (2 alpha^2) and the
+ * transition G(l) = log((fix e^l + chg) / (chg e^l + fix)), an odd function
+ * bounded by log(fix / chg).  The posterior of view v at a row is
+ * sigmoid(l_forward + l_backward).  Nothing underflows: the costs that the
+ * left/top-border tile-index quirk produces (|c| up to ~100) give E ~ -10^4,
+ * representable, where probabilities would be 0.  Constants keep their
+ * FP32 literal values. */
 struct MsgConst {
-    double up;      /* exp(-0.5 x^2 / alpha^2), x = 0.83f */
-    double inv2a2;  /* 0.5 / alpha^2 */
+    float inv2a2; /* 0.5 / alpha^2 */
+    float ex;     /* 0.5 x^2 / alpha^2, x = 0.83f */
 };
 
 __device__ __forceinline__ MsgConst msg_const(double alpha2) {
     MsgConst mc;
     const double x = (double)kUniformX;
-    mc.inv2a2 = 0.5 / alpha2;
-    mc.up = exp(-0.5 * x * x / alpha2);
+    mc.inv2a2 = (float)(0.5 / alpha2);
+    mc.ex = (float)(0.5 * x * x / alpha2);
     return mc;
 }
 
 /* Sentinel draws: every view whose stored cost is exactly 2.0 consumes one
- * uniform, in view order (src/patchMatch.cu:680-682, :706-707). */
-__device__ __forceinline__ double sentinel_emission(float cost, bool is_view, int lane, Xw& rng) {
+ * uniform, in view order, and uses 2u as its cost (src/patchMatch.cu:680-682,
+ * :706-707). */
+__device__ __forceinline__ float sentinel_emission(float cost, bool is_view, int lane, Xw& rng) {
     const bool sent = is_view && (cost == kSentinel);
     const uint64_t mask = __ballot(sent);
     const int k = __popcll(mask);
-    double e = (double)cost;
+    float e = cost;
     if (k) {
         const int rank = __popcll(mask & ((1ull << lane) - 1ull));
         for (int i = 0; i < k; i++) {
             const float u = xw_uniform(rng);
-            if (sent && rank == i) e = (double)u * 2.0;
+            if (sent && rank == i) e = u * 2.0f;
         }
     }
     return e;
 }
 
-/* exp(t) for t <= 0 in FP64 range: t log2(e) split into an integer n and a
- * fraction f in [-1/2, 1/2]; 2^f by the hardware FP32 exp2 (1 ulp), scaled
- * by 2^n in FP64.  Relative error ~1.5e-7 down to FP64's denormals, where
- * results round to 0 as an FP64 exp does. */
-__device__ __forceinline__ double exp_neg(double t) {
-    const double y = fmax(t * 1.4426950408889634, -2200.0);
-    const double n = rint(y);
-    const float f = (float)(y - n);
-    return ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
+/* E = log(e(c) / u) */
+__device__ __forceinline__ float emission_lr(float c, const MsgConst& mc) {
+    return fmaf(-(c * c), mc.inv2a2, mc.ex);
 }
 
-/* n / d for a probability, 0 <= n <= d: the hardware reciprocal refined by
- * two Newton steps and one correction of the quotient (within an ulp of
- * the quotient) where d is a normal number, a true divide where it is not:
- * with messages near 0 or 1 both terms can be denormal, where a reciprocal
- * overflows.  The wave takes the slow path only when one of its lanes needs
- * it (every lane must call this). */
-__device__ __forceinline__ double div_prob(double n, double d) {
-    if (__ballot(!(d >= 0x1p-1000)) == 0) {
-        double r = __builtin_amdgcn_rcp(d);
-        r = fma(r, fma(-d, r, 1.0), r);
-        r = fma(r, fma(-d, r, 1.0), r);
-        const double q = n * r;
-        return fma(r, fma(-d, q, n), q);
-    }
-    return n / d;
-}
-
-/* computeMessageBackward (src/patchMatch.cu:669-690) */
-__device__ __forceinline__ double msg_backward(double m, double e, const MsgConst& mc) {
-    const double fix = (double)kFix, chg = (double)kChange;
-    const double em = exp_neg(-e * e * mc.inv2a2);
-    const double z0 = m * em * fix + (1.0 - m) * mc.up * chg;
-    const double z1 = m * em * chg + (1.0 - m) * mc.up * fix;
-    return div_prob(z0, z0 + z1);
-}
-
-/* computeMessageForward (src/patchMatch.cu:692-715) */
-__device__ __forceinline__ double msg_forward(double m, double e, const MsgConst& mc) {
-    const double fix = (double)kFix, chg = (double)kChange;
-    const double em = exp_neg(-e * e * mc.inv2a2);
-    const double z0 = (m * fix + (1.0 - m) * chg) * em;
-    const double z1 = (m * chg + (1.0 - m) * fix) * mc.up;
-    return div_prob(z0, z0 + z1);
+/* G(l) = sign(l) log((fix + chg t) / (chg + fix t)), t = e^-|l| (no overflow) */
+__device__ __forceinline__ float trans_g(float l) {
+    const float t = __expf(-fabsf(l));
+    const float g = __logf(fmaf(kChange, t, kFix)) - __logf(fmaf(kFix, t, kChange));
+    return copysignf(g, l);
 }
 
 /* ---------------------------------------------------------------- sweep */
@@ -1142,15 +1114,16 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
 
     float* dcol = a.depth + (size_t)c * a.depth_ld;
     float* ccol = a.cost + (size_t)c * a.cost_ld;
-    double* scol = a.spm + (size_t)c * a.spm_ld;
+    float* scol = a.spm + (size_t)c * a.spm_ld;
 
     /* generator of this column: row 0 (plain) or column 0 (rotated) */
     pmx_rng_state* rp = a.rotated ? (pmx_rng_state*)((char*)a.rng + (size_t)c * a.rng_pitch)
                                   : a.rng + c;
     Xw rng = xw_uniformize(xw_load(rp));
 
-    /* ---- backward messages (src/patchMatch.cu:858-869, :1050-1061) ---- */
-    double m = 0.5;
+    /* ---- backward messages (src/patchMatch.cu:858-869, :1050-1061), as
+     * log-odds: m = 0.5 is l = 0 ---- */
+    float m = 0.0f;
     {
         int rb = a.down ? Hs_rows - 1 : 0;
         const int sbs = a.down ? -1 : 1;
@@ -1159,8 +1132,8 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             const float cst = cn;
             if (k + 1 < Hs_rows) cn = ccol[(size_t)(rb + sbs) * V + vl];
             if (lead && is_view) scol[(size_t)rb * V + lane] = m;
-            const double e = sentinel_emission(cst, is_view, lane, rng);
-            m = msg_backward(m, e, mc);
+            const float e = sentinel_emission(cst, is_view, lane, rng);
+            m = trans_g(m + emission_lr(e, mc));
         }
     }
     /* the backward messages are re-read below by every wave */
@@ -1171,8 +1144,8 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     const int r0 = a.down ? 0 : Hs_rows - 1;
     {
         const float cst = ccol[(size_t)r0 * V + vl];
-        const double e = sentinel_emission(cst, is_view, lane, rng);
-        m = msg_forward(0.5, e, mc);
+        const float e = sentinel_emission(cst, is_view, lane, rng);
+        m = emission_lr(e, mc); /* G(0) = 0 */
     }
     float d_former = dcol[r0];
 
@@ -1193,7 +1166,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     /* prefetch row r0 + step */
     int r = r0 + step;
     float pf_cost = 0.0f, pf_depth = 0.0f, pf_prop = 0.0f, pf_spread = 0.0f;
-    double pf_beta = 0.0;
+    float pf_beta = 0.0f;
     float2 pf_stat = make_float2(0.0f, 0.0f);
     float pf_win[PF];
     auto prefetch = [&](int rr) {
@@ -1234,7 +1207,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         float* win = winbuf + buf * WW;
         lds_barrier();
         const float cst = pf_cost, d_cur = pf_depth, spread = pf_spread;
-        const double beta = pf_beta;
+        const float beta = pf_beta;
         float d_prop = pf_prop;
         const float2 st = pf_stat;
         if (k + 1 < Hs_rows) prefetch(r + step);
@@ -1251,32 +1224,50 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
 
         /* forward message with the stored costs, posterior, CDF
          * (src/patchMatch.cu:901-923) */
-        const double m_prev = m;
-        double e = sentinel_emission(cst, is_view, lane, rng);
-        const double mf = msg_forward(m, e, mc);
-        double q;
+        const float g_prev = trans_g(m); /* shared with the refresh below */
+        const float mf = g_prev + emission_lr(sentinel_emission(cst, is_view, lane, rng), mc);
+        /* posterior log-odds of each view; the normalised CDF of its
+         * probabilities (inclusive prefix sum over lanes 0..V-1 divided by
+         * the total).  Where every posterior is below FP64's smallest
+         * number (e^-744.4, only at tile-index-quirk costs) the reference's
+         * arithmetic gives 0/0: the CDF is NaN and every draw takes the last
+         * view.  Below e^-80 the probabilities are scaled by the largest. */
+        float cdf;
         {
-            const double z0 = mf * beta;
-            const double z1 = (1.0 - mf) * (1.0 - beta);
-            q = div_prob(z0, z0 + z1);
-            q = is_view ? q : 0.0;
-        }
-        /* inclusive prefix sum over lanes 0..V-1, the normalised CDF of
-         * src/patchMatch.cu:917-923 */
-        double cdf = q;
-        if (V <= 16) {
-            cdf = row_scan16(cdf);
-        } else {
+            const float L = mf + beta;
+            float Lm = is_view ? L : -INFINITY;
+            if (V <= 16) {
+                Lm = row_max16(Lm);
+            } else {
 #pragma unroll
-            for (int o = 1; o < kLanes; o <<= 1) {
-                const double t = __shfl_up(cdf, o, kLanes);
-                if (lane >= o) cdf += t;
+                for (int o = 1; o < kLanes; o <<= 1) Lm = fmaxf(Lm, __shfl_xor(Lm, o, kLanes));
             }
-        }
-        const double total = lane_d(cdf, V - 1);
-        {
-            const double nc = div_prob(cdf, total);
-            cdf = (lane == V - 1) ? 1.0 : nc;
+            const float Lmax = lane_f(Lm, 0);
+            float q = (Lmax > -80.0f) ? __builtin_amdgcn_rcpf(1.0f + __expf(-L)) : __expf(L - Lmax);
+            q = is_view ? q : 0.0f;
+            cdf = q;
+            if (V <= 16) {
+                cdf = row_scan16f(cdf);
+            } else {
+#pragma unroll
+                for (int o = 1; o < kLanes; o <<= 1) {
+                    const float t = __shfl_up(cdf, o, kLanes);
+                    if (lane >= o) cdf += t;
+                }
+            }
+            const float total = lane_f(cdf, V - 1);
+            cdf = (lane == V - 1) ? 1.0f : cdf * __builtin_amdgcn_rcpf(total);
+            if (Lmax < -700.0f) {
+                /* the reference's numerator m_f * beta of each view, in log */
+                float U = is_view ? mf - __logf(1.0f + __expf(-beta)) : -INFINITY;
+                if (V <= 16) {
+                    U = row_max16(U);
+                } else {
+#pragma unroll
+                    for (int o = 1; o < kLanes; o <<= 1) U = fmaxf(U, __shfl_xor(U, o, kLanes));
+                }
+                if (lane_f(U, 0) < -744.4f) cdf = __builtin_nanf("");
+            }
         }
 
         /* draws: random depth, then S views (src/patchMatch.cu:937-961).
@@ -1289,7 +1280,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         {
             const int per_b = kLanes / V;
             const int ld = lane / V, lv = lane - ld * V;
-            const double cv = __shfl(cdf, lv, kLanes);
+            const float cv = __shfl(cdf, lv, kLanes);
             const uint64_t vmask = (V >= 64) ? ~0ull : ((1ull << V) - 1ull);
             for (int j0 = 0; j0 < a.S; j0 += per_b) {
                 const int nd = min(per_b, a.S - j0);
@@ -1298,7 +1289,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                     const float u = xw_uniform(rng);
                     uj = (ld == t) ? u : uj;
                 }
-                const uint64_t hit = __ballot(ld < nd && (double)uj <= cv);
+                const uint64_t hit = __ballot(ld < nd && uj <= cv);
                 for (int t = 0; t < nd; t++) {
                     const uint64_t f = (hit >> (t * V)) & vmask;
                     const int id = f ? (int)__builtin_ctzll(f) : V - 1;
@@ -1405,8 +1396,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         if (idx == 1 && !__ballot(is_view && cst == kSentinel)) {
             m = mf;
         } else {
-            e = sentinel_emission(cnew, is_view, lane, rng);
-            m = msg_forward(m_prev, e, mc);
+            m = g_prev + emission_lr(sentinel_emission(cnew, is_view, lane, rng), mc);
         }
         d_former = best;
 
