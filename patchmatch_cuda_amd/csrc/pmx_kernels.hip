@@ -657,6 +657,73 @@ __device__ __forceinline__ void accum_rows(const SB& sb, const Hyp& q, int Wn, f
     }
 }
 
+/* Value of lane n of each 16-lane row in every lane of the row
+ * (DPP row_newbcast:n). */
+template <int N>
+__device__ __forceinline__ float row_bcast16(float v) {
+    return dpp_f<0x150 + N>(v);
+}
+
+/* accum_rows for a whole window column (R = Wn rows, r0 = 0) of a pixel
+ * whose window rows are all inside the image, with 16-lane groups.  No
+ * per-tap masks: lanes whose window column is outside (and the 16th lane)
+ * are zeroed by the caller after the loop.  The centring sample K is the
+ * window's own centre tap (row R/2 of lane R/2 of the group), broadcast
+ * within the group, instead of a separate fetch.  It is fetched with the
+ * first batch (so every batch size accumulates the rows in the same order
+ * with the same K). */
+template <int R, int NB, class SB>
+__device__ __forceinline__ void accum_rows_interior(const SB& sb, const Hyp& q, int Wn, float dxf,
+                                                    const float* __restrict__ wp, float& K,
+                                                    float& A, float& B, float& C) {
+    constexpr int HC = R / 2;
+    constexpr float dy0 = -(float)HC;
+    const float nxa = fmaf(q.uy, dy0, q.ux * dxf);
+    const float nya = fmaf(q.vy, dy0, q.vx * dxf);
+    const float dza = fmaf(q.cy, dy0, fmaf(q.cx, dxf, 1.0f));
+    auto pos = [&](int i) -> float2 {
+        const float fi = (float)i;
+        const float rz = __builtin_amdgcn_rcpf(fmaf(q.cy, fi, dza));
+        return make_float2(fmaf(fmaf(q.uy, fi, nxa), rz, q.fxc), fmaf(fmaf(q.vy, fi, nya), rz, q.fyc));
+    };
+    typename SB::Tex k0, k1;
+    TapW kw;
+    float kv = 0.0f; /* this lane's row-HC sample */
+#pragma unroll
+    for (int b0 = 0; b0 < R; b0 += NB) {
+        typename SB::Tex t0[NB], t1[NB];
+        TapW tw[NB];
+        if (b0 == 0) {
+            const float2 p = pos(HC);
+            kw = tap_fetch(sb, q, p.x, p.y, k0, k1);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const int i = b0 + j;
+            if (i < R && i != HC) {
+                const float2 p = pos(i);
+                tw[j] = tap_fetch(sb, q, p.x, p.y, t0[j], t1[j]);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (b0 == 0) {
+            kv = tap_blend(kw, k0, k1);
+            K = row_bcast16<HC>(kv);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const int i = b0 + j;
+            if (i < R) {
+                const float val = (i == HC) ? kv : tap_blend(tw[j], t0[j], t1[j]);
+                const float e = val - K;
+                A += e;
+                B = fmaf(e, e, B);
+                C = fmaf(e, wp[i * Wn], C);
+            }
+        }
+    }
+}
+
 /* 1 - NCC from the centred sums (src/patchMatch.cu:594-610), final
  * normalisation in FP64:  n var' = B - A^2/n,
  * sum I' (Rq - mean) = C + K D  (D = sum (Rq - mean), 0 unless the
@@ -689,7 +756,7 @@ __device__ __forceinline__ float ncc_finish(float A, float B, float C, float K,
  * sums to part[t * NW + slice] and the caller combines them after a
  * barrier (combine_pass).  Returns k.
  */
-template <int G, int WN, int NW, int NB = 0, class SB>
+template <int G, int WN, int NW, int NB = 0, bool INT = false, class SB>
 __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __restrict__ Hm,
                                          const SB& sb, const float* __restrict__ win,
                                          int wave, int lane, int nt, const int* tview,
@@ -702,15 +769,17 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
      * which must fit 512 / (2 NW) VGPRs; NB > 0 sets it */
     constexpr int kNB = NB > 0 ? NB : (NW == 1 ? 16 : 8);
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
-    int k = 1;
-    while (2 * k * nt <= NGT) k *= 2;
+    /* k = NGT >> ceil(log2(nt)): the largest power of two with k nt <= NGT
+     * (1 <= nt <= NGT); shifts instead of divisions by k */
+    const int lk = __builtin_ctz((unsigned)NGT) - (nt > 1 ? 32 - __builtin_clz((unsigned)(nt - 1)) : 0);
+    const int k = 1 << lk;
     const int gg = wave * NG + lane / G, gl = lane % G;
-    const int t = gg / k, sub = gg - t * k;
+    const int t = gg >> lk, sub = gg & (k - 1);
     const bool active = t < nt;
     float A = 0.0f, B = 0.0f, C = 0.0f, K = 0.0f;
     if (active) {
         const Hyp q = make_hyp(Hm, sb, tview[t], tdep[t], pc);
-        const int R = (Wn + k - 1) / k;
+        const int R = (Wn + k - 1) >> lk;
         const int r0 = max(sub * R, pc.dy_lo);
         const int r1 = min(min((sub + 1) * R, Wn), pc.dy_hi + 1);
         const int nr = r1 - r0;
@@ -720,7 +789,18 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
         const float dxf = (float)(gl - a.hw);
         const float dy0 = (float)(r0 - a.hw);
         const float* wp = win + r0 * Wn + gl;
-        if constexpr (WN > 0) {
+        bool done = false;
+        if constexpr (INT && G == 16 && WN > 0) {
+            /* a whole window column per group, every window row inside the
+             * image (wave-uniform: one pixel per wave) */
+            if (k == 1 && pc.dy_lo == 0 && pc.dy_hi == Wn - 1) {
+                accum_rows_interior<WN, cmin(WN, kNB)>(sb, q, Wn, dxf, win + gl, K, A, B, C);
+                if (!pc.lane_in) A = B = C = 0.0f;
+                done = true;
+            }
+        }
+        if (done) {
+        } else if constexpr (WN > 0) {
             switch (k) {
             case 1: accum_rows<WN, cmin(WN, kNB)>(sb, q, Wn, dxf, dy0, wp, nr, valid, K, A, B, C); break;
             case 2: accum_rows<(WN + 1) / 2, cmin((WN + 1) / 2, kNB)>(sb, q, Wn, dxf, dy0, wp, nr, valid, K, A, B, C); break;
@@ -782,22 +862,22 @@ __device__ __forceinline__ void combine_pass(int lane, int nt, int slices, const
 template <int PF>
 __device__ __forceinline__ void load_window(const PmxSweepArgs& a, int x, int y, int qx, int qy,
                                             int Wn, float inv_wn, int lane, float* vals) {
+    /* branch-free: one raw buffer load per position (positions that read
+     * nothing load element 0 and discard it) */
+    const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)a.ref, (short)0, a.ref_pitch * a.H * (int)sizeof(float), 0x00020000);
 #pragma unroll
     for (int k = 0; k < PF; k++) {
         const int idx = lane + kLanes * k;
-        float v = 0.0f;
-        if (idx < Wn * Wn) {
-            const int dyi = (int)(((float)idx + 0.5f) * inv_wn);
-            const int dxi = idx - dyi * Wn;
-            const int px = x + dxi - a.hw, py = y + dyi - a.hw;
-            if (px >= 0 && px < a.W && py >= 0 && py < a.H) {
-                const int sx = px - qx, sy = py - qy;
-                v = (sx >= 0 && sy >= 0) ? a.ref[(size_t)sy * a.ref_pitch + sx] : 0.0f;
-            } else {
-                v = __builtin_nanf("");
-            }
-        }
-        vals[k] = v;
+        const int dyi = (int)(((float)idx + 0.5f) * inv_wn);
+        const int dxi = idx - dyi * Wn;
+        const int px = x + dxi - a.hw, py = y + dyi - a.hw;
+        const bool in = (unsigned)px < (unsigned)a.W && (unsigned)py < (unsigned)a.H;
+        const int sx = px - qx, sy = py - qy;
+        const bool rd = in && (sx | sy) >= 0 && idx < Wn * Wn;
+        const uint32_t off = rd ? (uint32_t)(sy * a.ref_pitch + sx) * 4u : 0u;
+        const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, off, 0, 0));
+        vals[k] = in ? (rd ? v : 0.0f) : __builtin_nanf("");
     }
 }
 
@@ -838,7 +918,7 @@ __device__ __forceinline__ void pixel_ctx(const PmxSweepArgs& a, int x, int y, i
     const int nx = min(x + hw, a.W - 1) - max(x - hw, 0) + 1;
     const int ny = pc.dy_hi - pc.dy_lo + 1;
     pc.npx = nx * ny;
-    pc.inv_n = 1.0 / (double)pc.npx;
+    pc.inv_n = rcp64((double)pc.npx);
     pc.cx = (double)x + 0.5;
     pc.cy = (double)y + 0.5;
     pc.scale = a.oX * (a.iK0 * pc.cx + a.iK1) + a.oZ;
@@ -1195,7 +1275,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         for (int t0 = 0; t0 < nt; t0 += NGT) {
             const int ntp = min(NGT, nt - t0);
             float4* pp = part + (pb & 1) * 64 * NW;
-            const int kk = eval_pass<G, WN, NW, NB>(a, Hs, sb, win, wave, lane, ntp, tview + t0,
+            const int kk = eval_pass<G, WN, NW, NB, true>(a, Hs, sb, win, wave, lane, ntp, tview + t0,
                                                 tdep + t0, tout + t0, pp, pc);
             lds_barrier();
             if (kk > NG) combine_pass<NW>(lane, ntp, kk / NG, pp, tout + t0, pc);
@@ -1272,12 +1352,38 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
 
         /* draws: random depth, then S views (src/patchMatch.cu:937-961).
          * A view draw takes the first view whose CDF entry is >= u (none:
-         * a NaN CDF, the last view).  floor(64 / V) draws are resolved by one
-         * ballot: lane d V + i compares draw d with view i's CDF entry. */
-        const float d_rand = xw_uniform(rng) * (a.far_d - a.near_d) + a.near_d;
+         * a NaN CDF, the last view). */
+        float d_rand;
         uint64_t sel = 0;
-        int cnt = 0; /* draws that picked this lane's view */
-        {
+        int cnt = 0;   /* draws that picked this lane's view */
+        int my_id = 0; /* view of draw `lane` (draws 0..63) */
+        if (a.S < 8) {
+            /* The row's 1 + S draws unrolled, so the generator's word
+             * rotation is register renaming.  u = k 2^-24 with the integer
+             * k = (r >> 8) + 1, so u <= cdf is exactly k <= floor(cdf 2^24),
+             * an integer compare whose lane mask is the draw's hits; lane
+             * V - 1 holds 2^24 (its CDF entry is 1) unless the CDF is NaN,
+             * where every entry converts to 0 and the draw takes view V - 1
+             * through the bit set below. */
+            const uint32_t k0 = (xw_next(rng) >> 8) + 1u;
+            d_rand = (float)k0 * 5.9604644775390625e-08f * (a.far_d - a.near_d) + a.near_d;
+            const uint32_t cvi = (is_view && cdf == cdf) ? (uint32_t)(cdf * 16777216.0f) : 0u;
+            const uint64_t last = 1ull << (V - 1);
+#pragma unroll
+            for (int t = 1; t < 8; t++) {
+                if (t <= a.S) {
+                    const uint32_t kt = (xw_next(rng) >> 8) + 1u;
+                    const uint64_t hit = __ballot(kt <= cvi) | last;
+                    const int id = (int)__builtin_ctzll(hit);
+                    sel |= 1ull << id;
+                    cnt += (lane == id) ? 1 : 0;
+                    my_id = (lane == t - 1) ? id : my_id;
+                }
+            }
+        } else {
+            /* floor(64 / V) draws are resolved by one ballot: lane d V + i
+             * compares draw d with view i's CDF entry */
+            d_rand = xw_uniform(rng) * (a.far_d - a.near_d) + a.near_d;
             const int per_b = kLanes / V;
             const int ld = lane / V, lv = lane - ld * V;
             const float cv = __shfl(cdf, lv, kLanes);
@@ -1295,11 +1401,15 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                     const int id = f ? (int)__builtin_ctzll(f) : V - 1;
                     sel |= 1ull << id;
                     cnt += (lane == id) ? 1 : 0;
-                    if (lead && a.ids && lane == 0)
+                    my_id = (lane == j0 + t) ? id : my_id;
+                    if (j0 + t >= kLanes && lead && a.ids && lane == 0)
                         a.ids[((size_t)(j0 + t) * a.H + y) * a.ids_pitch + x] = (uint8_t)id;
                 }
             }
         }
+        /* the ids of draws 0..63, one per lane, in one store */
+        if (lead && a.ids && lane < a.S)
+            a.ids[((size_t)lane * a.H + y) * a.ids_pitch + x] = (uint8_t)my_id;
         const int U = __popcll(sel);
         const bool in_sel = is_view && ((sel >> lane) & 1ull);
         const int srank = __popcll(sel & ((1ull << lane) - 1ull));
