@@ -235,9 +235,10 @@ int pmx_engine_export_state(pmx_engine* eng, float** ref, float** sum_I,
                             int* rng_pitch);
 int pmx_engine_set_timing(pmx_engine* eng, int enabled);
 /* Launch shape of the sweep kernel: wavefronts per sweep column (1, 2 or
- * 4), or 0 = automatic, the default (two waves per column where every
- * column still fits three waves per SIMD, e.g. the 1536-column sweeps of a
- * 2048x1536 view, for windows 11 and 15; one elsewhere).  No reference counterpart (its sweeps use one 32-thread
+ * 4), or 0 = automatic, the default (two waves per column where the
+ * columns fill at most half of the device's SIMDs, e.g. a 640x480 view;
+ * one elsewhere, e.g. both sweep orientations of a 2048x1536 view).  No
+ * reference counterpart (its sweeps use one 32-thread
  * block per 32 columns, src/patchMatch.cu:334).  Results do not depend on
  * it beyond floating-point summation order. */
 int pmx_engine_set_sweep_waves(pmx_engine* eng, int waves);

@@ -298,8 +298,14 @@ int do_cost_all(pmx_engine* e, int rotated, Timer* tm) {
 }
 
 /* Launch shape of one sweep: the engine's setting, or with the automatic
- * setting (0) two waves per column when the columns then still fit three
- * waves per SIMD (every column resident at once), else one. */
+ * setting (0) two waves per column (at three waves per SIMD) when the
+ * columns fill at most half of the SIMDs, else one.  A SIMD issues about
+ * one instruction every 4 cycles over all of its waves, and a second wave
+ * per column repeats the column's sequential logic: where the columns
+ * already put one or two waves on every SIMD it only adds instructions
+ * (1536-column sweeps of a 2048x1536 view: 16.9 ms per launch with one
+ * wave per column, 17.7 ms with two).  Below that the second wave runs
+ * on an otherwise idle SIMD and halves each row's hypothesis passes. */
 int sweep_waves_for(pmx_engine* e, int Ws) {
     if (e->sweep_waves > 0) return e->sweep_waves;
     if (e->num_cus <= 0) {
@@ -309,7 +315,7 @@ int sweep_waves_for(pmx_engine* e, int Ws) {
             cus = 256;
         e->num_cus = cus;
     }
-    return 2 * Ws <= 3 * 4 * e->num_cus ? 3 : 1;
+    return Ws <= 2 * e->num_cus ? 3 : 1;
 }
 
 int do_sweep(pmx_engine* e, int down, int rotated, int S, bool ids, Timer* tm) {

@@ -1358,6 +1358,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         int cnt = 0;   /* draws that picked this lane's view */
         int my_id = 0; /* view of draw `lane` (draws 0..63) */
         if (a.S < 8) {
+            uint64_t ids6 = 0; /* the view ids, 6 bits per draw */
             /* The row's 1 + S draws unrolled, so the generator's word
              * rotation is register renaming.  u = k 2^-24 with the integer
              * k = (r >> 8) + 1, so u <= cdf is exactly k <= floor(cdf 2^24),
@@ -1377,9 +1378,10 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                     const int id = (int)__builtin_ctzll(hit);
                     sel |= 1ull << id;
                     cnt += (lane == id) ? 1 : 0;
-                    my_id = (lane == t - 1) ? id : my_id;
+                    ids6 |= (uint64_t)id << (6 * (t - 1));
                 }
             }
+            my_id = (int)((ids6 >> (6 * min(lane, 7))) & 63u);
         } else {
             /* floor(64 / V) draws are resolved by one ballot: lane d V + i
              * compares draw d with view i's CDF entry */

@@ -207,9 +207,9 @@ def test_full_run_c1_depth_agreement(c1):
 
 def test_full_run_c1_one_wave(c1):
     """One wave per column everywhere (the default automatic shape runs two
-    waves per column at three waves per SIMD wherever the columns fit, as on
-    every sweep of this 320x240 view) meets the same bar, and its
-    teacher-forced sweep agrees decision for decision."""
+    waves per column at three waves per SIMD where the columns fill at most
+    half of the SIMDs, as on every sweep of this 320x240 view) meets the
+    same bar, and its teacher-forced sweep agrees decision for decision."""
     spec, rp, sc, eng = c1
     _say("case: full run c1, one wave per sweep column")
     eng.set_sweep_waves(1)
