@@ -89,7 +89,7 @@ class Server:
         self.max_batch = max_batch
         self.nets = {}
         self.resident = 0
-        self.keep_results = False
+        self.keep_results = False   # True: every depth map; a set: those request ids
         self.results = {}
         self.launch_shapes = []   # batch launch shape of every group of the last run
 
@@ -155,7 +155,8 @@ class Server:
         px = 0
         for req, fp, e in st["admitted"]:
             d = e.depth()                # synchronises the group's stream
-            if self.keep_results:
+            keep = self.keep_results
+            if keep is True or (keep and req.rid in keep):
                 self.results[req.rid] = d
             lat.append(time.perf_counter() - t0 - req.arrival)
             W, H = self.dims(req)
