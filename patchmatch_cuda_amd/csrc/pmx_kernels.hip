@@ -99,6 +99,11 @@ __device__ __forceinline__ float wave_sum(float v, int n) {
     return s;
 }
 
+/* A wave-uniform value in a scalar register (its load is waited for here). */
+__device__ __forceinline__ float uniform_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
 __device__ __forceinline__ float lane_f(float v, int src) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
 }
@@ -855,17 +860,29 @@ __device__ __forceinline__ void combine_pass(int lane, int nt, int slices, const
     }
 }
 
-/* Loads the reference window of pixel (x, y) for one lane's share:
- * positions idx = lane + 64*k.  Returns R' (or 0 outside) and adds the
- * in-bounds values to *dsum. qx/qy: tile-index shift of the left (plain) or
- * top (rotated) border (src/patchMatch.cu:536-551). */
+/* One lane's share of a reference window, positions idx = lane + 64*k: the
+ * loaded values as they arrive, and per position k bit k of `in` (inside
+ * the image) and of `rd` (inside and read; else the value is 0).  Nothing
+ * looks at a value before store_window, so a window prefetched a row ahead
+ * is not waited for until it is stored. */
+template <int PF>
+struct WinVals {
+    float v[PF];
+    uint32_t in, rd;
+};
+
+/* Loads the reference window of pixel (x, y) for one lane's share.
+ * qx/qy: tile-index shift of the left (plain) or top (rotated) border
+ * (src/patchMatch.cu:536-551). */
 template <int PF>
 __device__ __forceinline__ void load_window(const PmxSweepArgs& a, int x, int y, int qx, int qy,
-                                            int Wn, float inv_wn, int lane, float* vals) {
+                                            int Wn, float inv_wn, int lane, WinVals<PF>& w) {
     /* branch-free: one raw buffer load per position (positions that read
-     * nothing load element 0 and discard it) */
+     * nothing load element 0, which is discarded) */
     const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
         (void*)a.ref, (short)0, a.ref_pitch * a.H * (int)sizeof(float), 0x00020000);
+    w.in = 0u;
+    w.rd = 0u;
 #pragma unroll
     for (int k = 0; k < PF; k++) {
         const int idx = lane + kLanes * k;
@@ -876,26 +893,26 @@ __device__ __forceinline__ void load_window(const PmxSweepArgs& a, int x, int y,
         const int sx = px - qx, sy = py - qy;
         const bool rd = in && (sx | sy) >= 0 && idx < Wn * Wn;
         const uint32_t off = rd ? (uint32_t)(sy * a.ref_pitch + sx) * 4u : 0u;
-        const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, off, 0, 0));
-        vals[k] = in ? (rd ? v : 0.0f) : __builtin_nanf("");
+        w.v[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, off, 0, 0));
+        w.in |= (in ? 1u : 0u) << k;
+        w.rd |= (rd ? 1u : 0u) << k;
     }
 }
 
-/* Writes a loaded window as R' = Rq - mean into LDS.  With need_d (the
- * tile-index shift applies) returns the wave sum D of R' over the in-bounds
- * positions; otherwise D is exactly 0 (the in-bounds values sum to the box
- * sum) and 0 is returned. */
+/* Writes a loaded window as R' = Rq - mean into LDS (0 outside the image).
+ * With need_d (the tile-index shift applies) returns the wave sum D of R'
+ * over the in-bounds positions; otherwise D is exactly 0 (the in-bounds
+ * values sum to the box sum) and 0 is returned. */
 template <int PF>
-__device__ __forceinline__ float store_window(float* __restrict__ win, const float* vals, float mu,
-                                              int Wn, int lane, bool need_d) {
+__device__ __forceinline__ float store_window(float* __restrict__ win, const WinVals<PF>& w,
+                                              float mu, int Wn, int lane, bool need_d) {
     float d = 0.0f;
 #pragma unroll
     for (int k = 0; k < PF; k++) {
         const int idx = lane + kLanes * k;
         if (idx < Wn * Wn) {
-            const float v = vals[k];
-            const bool in = (v == v);
-            const float r = in ? v - mu : 0.0f;
+            const float v = ((w.rd >> k) & 1u) ? w.v[k] : 0.0f;
+            const float r = ((w.in >> k) & 1u) ? v - mu : 0.0f;
             win[idx] = r;
             d += r;
         }
@@ -972,7 +989,7 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
      * along y (src/patchMatch.cu:794). */
     const int qx = a.rotated ? 0 : max(0, a.hw - x);
     const int qy = a.rotated ? max(0, a.hw - y) : 0;
-    float vals[PF];
+    WinVals<PF> vals;
     load_window<PF>(a, x, y, qx, qy, Wn, inv_wn, lane, vals);
     pc.D = store_window<PF>(win, vals, pc.mu, Wn, lane, qx != 0 || qy != 0);
     /* state is row-major: column c = y, row r = x */
@@ -1024,7 +1041,7 @@ __global__ __launch_bounds__(64) void pmx_volume_kernel(PmxSweepArgs a, int view
     pc.mu = st.x;
     pc.c1 = st.y;
     const int qx = max(0, a.hw - x);
-    float vals[PF];
+    WinVals<PF> vals;
     load_window<PF>(a, x, y, qx, 0, Wn, inv_wn, lane, vals);
     pc.D = store_window<PF>(win, vals, pc.mu, Wn, lane, qx != 0);
     {
@@ -1166,7 +1183,7 @@ __device__ __forceinline__ float proposal_draw(float depth, float spread, int x,
  * (the reference recomputes the same value).  Wave 0 alone writes the
  * column's outputs.
  */
-template <int G, int PF, int WN, int NW, bool H16, int NB = 0>
+template <int G, int PF, int WN, int NW, bool H16, int NB = 0, bool SM = false>
 __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c) {
     extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
@@ -1219,16 +1236,8 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     /* the backward messages are re-read below by every wave */
     __threadfence_block();
     __syncthreads();
-    /* ---- forward message at the first row ---- */
     const int step = a.down ? 1 : -1;
     const int r0 = a.down ? 0 : Hs_rows - 1;
-    {
-        const float cst = ccol[(size_t)r0 * V + vl];
-        const float e = sentinel_emission(cst, is_view, lane, rng);
-        m = emission_lr(e, mc); /* G(0) = 0 */
-    }
-    float d_former = dcol[r0];
-
     /* original-frame coordinates of sweep row r in this column */
     auto orig_xy = [&](int r, int& x, int& y) {
         if (a.rotated) {
@@ -1239,6 +1248,14 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             y = r;
         }
     };
+    /* ---- forward message at the first row ---- */
+    {
+        const float cst = ccol[(size_t)r0 * V + vl];
+        const float e = sentinel_emission(cst, is_view, lane, rng);
+        m = emission_lr(e, mc); /* G(0) = 0 */
+    }
+    float d_former = dcol[r0];
+
     const int qx_col = a.rotated ? 0 : max(0, a.hw - c);
     const int qy_col = a.rotated ? max(0, a.hw - c) : 0;
     const bool need_d = qx_col != 0 || qy_col != 0;
@@ -1248,7 +1265,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     float pf_cost = 0.0f, pf_depth = 0.0f, pf_prop = 0.0f, pf_spread = 0.0f;
     float pf_beta = 0.0f;
     float2 pf_stat = make_float2(0.0f, 0.0f);
-    float pf_win[PF];
+    WinVals<PF> pf_win;
     auto prefetch = [&](int rr) {
         int x, y;
         orig_xy(rr, x, y);
@@ -1286,11 +1303,14 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     for (int k = 1; k < Hs_rows; k++, r += step) {
         float* win = winbuf + buf * WW;
         lds_barrier();
-        const float cst = pf_cost, d_cur = pf_depth, spread = pf_spread;
-        const float beta = pf_beta;
-        float d_prop = pf_prop;
-        const float2 st = pf_stat;
-        if (k + 1 < Hs_rows) prefetch(r + step);
+        /* This row's values, prefetched one row ago, are consumed before
+         * the next row's loads go out: a first use after those loads would
+         * wait for them as well (memory counters drain in order).  The
+         * column-uniform ones move to scalar registers. */
+        const float cst = pf_cost, beta = pf_beta;
+        const float d_cur = uniform_f(pf_depth), spread = uniform_f(pf_spread);
+        float d_prop = uniform_f(pf_prop);
+        const float2 st = make_float2(uniform_f(pf_stat.x), uniform_f(pf_stat.y));
 
         int x, y;
         orig_xy(r, x, y);
@@ -1301,6 +1321,15 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         pc.c1 = st.y;
         pc.D = D;
         const int npx = pc.npx;
+        /* The former depth equal to the current one: its NCC on a view is
+         * the stored cost of that view (the same function of the same
+         * pixel, depth and images), so former and current tie exactly and
+         * findMinCost keeps the current depth unless the random (or
+         * proposal) candidate is at least as good.  Only the left and top
+         * border columns, where the reference's tile-index shift makes the
+         * NCC depend on the sweep frame the stored cost came from, evaluate
+         * the former depth anyway. */
+        const bool same = d_former == d_cur && x >= a.hw && y >= a.hw;
 
         /* forward message with the stored costs, posterior, CDF
          * (src/patchMatch.cu:901-923) */
@@ -1316,7 +1345,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         {
             const float L = mf + beta;
             float Lm = is_view ? L : -INFINITY;
-            if (V <= 16) {
+            if (SM || V <= 16) {
                 Lm = row_max16(Lm);
             } else {
 #pragma unroll
@@ -1326,7 +1355,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             float q = (Lmax > -80.0f) ? __builtin_amdgcn_rcpf(1.0f + __expf(-L)) : __expf(L - Lmax);
             q = is_view ? q : 0.0f;
             cdf = q;
-            if (V <= 16) {
+            if (SM || V <= 16) {
                 cdf = row_scan16f(cdf);
             } else {
 #pragma unroll
@@ -1340,7 +1369,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             if (Lmax < -700.0f) {
                 /* the reference's numerator m_f * beta of each view, in log */
                 float U = is_view ? mf - __logf(1.0f + __expf(-beta)) : -INFINITY;
-                if (V <= 16) {
+                if (SM || V <= 16) {
                     U = row_max16(U);
                 } else {
 #pragma unroll
@@ -1350,6 +1379,10 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             }
         }
 
+        /* the next row's prefetch, once this row's values are in registers */
+        __builtin_amdgcn_sched_barrier(0);
+        if (k + 1 < Hs_rows) prefetch(r + step);
+
         /* draws: random depth, then S views (src/patchMatch.cu:937-961).
          * A view draw takes the first view whose CDF entry is >= u (none:
          * a NaN CDF, the last view). */
@@ -1357,7 +1390,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         uint64_t sel = 0;
         int cnt = 0;   /* draws that picked this lane's view */
         int my_id = 0; /* view of draw `lane` (draws 0..63) */
-        if (a.S < 8) {
+        if (SM || a.S < 8) {
             uint64_t ids6 = 0; /* the view ids, 6 bits per draw */
             /* The row's 1 + S draws unrolled, so the generator's word
              * rotation is register renaming.  u = k 2^-24 with the integer
@@ -1370,16 +1403,21 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             d_rand = (float)k0 * 5.9604644775390625e-08f * (a.far_d - a.near_d) + a.near_d;
             const uint32_t cvi = (is_view && cdf == cdf) ? (uint32_t)(cdf * 16777216.0f) : 0u;
             const uint64_t last = 1ull << (V - 1);
+            auto draw = [&](int t) {
+                const uint32_t kt = (xw_next(rng) >> 8) + 1u;
+                const uint64_t hit = __ballot(kt <= cvi) | last;
+                const int id = (int)__builtin_ctzll(hit);
+                sel |= 1ull << id;
+                cnt += (lane == id) ? 1 : 0;
+                ids6 |= (uint64_t)id << (6 * (t - 1));
+            };
+            if constexpr (SM) {
+                /* a loop: no per-draw predicates held across the row loop */
+                for (int t = 1; t <= a.S; t++) draw(t);
+            } else {
 #pragma unroll
-            for (int t = 1; t < 8; t++) {
-                if (t <= a.S) {
-                    const uint32_t kt = (xw_next(rng) >> 8) + 1u;
-                    const uint64_t hit = __ballot(kt <= cvi) | last;
-                    const int id = (int)__builtin_ctzll(hit);
-                    sel |= 1ull << id;
-                    cnt += (lane == id) ? 1 : 0;
-                    ids6 |= (uint64_t)id << (6 * (t - 1));
-                }
+                for (int t = 1; t < 8; t++)
+                    if (t <= a.S) draw(t);
             }
             my_id = (int)((ids6 >> (6 * min(lane, 7))) & 63u);
         } else {
@@ -1417,15 +1455,6 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         const int srank = __popcll(sel & ((1ull << lane) - 1ull));
         /* a learned proposal is a fourth candidate on the same views */
         const bool has_prop = d_prop > 0.0f;
-        /* The former depth equal to the current one: its NCC on a view is
-         * the stored cost of that view (the same function of the same
-         * pixel, depth and images), so former and current tie exactly and
-         * findMinCost keeps the current depth unless the random (or
-         * proposal) candidate is at least as good.  Only the left and top
-         * border columns, where the reference's tile-index shift makes the
-         * NCC depend on the sweep frame the stored cost came from, evaluate
-         * the former depth anyway. */
-        const bool same = d_former == d_cur && x >= a.hw && y >= a.hw;
         const int per = (same ? 1 : 2) + (has_prop ? 1 : 0);
         if (in_sel) {
             int o = per * srank;
@@ -1440,10 +1469,10 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                 tdep[o] = d_prop;
             }
         }
+        const int nt = per * U;
         if (NW == 1) lds_barrier();
 
         /* phase B: former and random depth (and the proposal) on the sampled views */
-        const int nt = per * U;
         run_phase(nt, win, tres, pc);
         float nf = 0.0f, nr = 0.0f, np = 0.0f;
         if (in_sel) {
@@ -1457,10 +1486,11 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         float c0, c1s, c2, c3 = 0.0f;
         {
             const float k = is_view ? (float)cnt : 0.0f;
-            c0 = wave_sum(is_view ? k * nf : 0.0f, V);
-            c1s = wave_sum(is_view ? k * cst : 0.0f, V);
-            c2 = wave_sum(is_view ? k * nr : 0.0f, V);
-            if (has_prop) c3 = wave_sum(is_view ? k * np : 0.0f, V);
+            auto vsum = [&](float v) { return SM ? lane_f(row_sum16(v), 0) : wave_sum(v, V); };
+            c0 = vsum(is_view ? k * nf : 0.0f);
+            c1s = vsum(is_view ? k * cst : 0.0f);
+            c2 = vsum(is_view ? k * nr : 0.0f);
+            if (has_prop) c3 = vsum(is_view ? k * np : 0.0f);
         }
         const float inv_s = 1.0f / (float)a.S;
         c0 *= inv_s;
@@ -1535,12 +1565,12 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
  * (WPE = 3), which leaves 168 registers instead of 128; its taps go out in
  * batches of 6 rows, which fit them without spilling (8-row batches spill
  * 6 registers and take 1.3% longer per 2048x1536 run). */
-template <int G, int PF, int WN, int NW, bool H16, int WPE>
+template <int G, int PF, int WN, int NW, bool H16, int WPE, bool SM = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE)))
 void pmx_sweep_kernel(PmxSweepArgs a) {
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= a.Ws) return;
-    sweep_column<G, PF, WN, NW, H16, (NW == 2 && WPE == 3) ? 6 : 0>(a, c);
+    sweep_column<G, PF, WN, NW, H16, (NW == 2 && WPE == 3) ? 6 : 0, SM>(a, c);
 }
 
 /*
@@ -1712,17 +1742,17 @@ hipError_t launch_volume_t(hipStream_t s, const PmxSweepArgs& a, int v0, int v1,
     return hipGetLastError();
 }
 
-template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW)>
+template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW), bool SM = false>
 hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
     const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * 2 * 64 * NW +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     if (a.src_h16)
-        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, true, WPE>), dim3(a.Ws), dim3(64 * NW),
-                           sh, s, a);
+        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, true, WPE, SM>), dim3(a.Ws),
+                           dim3(64 * NW), sh, s, a);
     else
-        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, false, WPE>), dim3(a.Ws), dim3(64 * NW),
-                           sh, s, a);
+        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, false, WPE, SM>), dim3(a.Ws),
+                           dim3(64 * NW), sh, s, a);
     return hipGetLastError();
 }
 template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW)>
@@ -1828,6 +1858,14 @@ hipError_t pmx_launch_seed_depth(hipStream_t s, const float* prop, int prop_ld, 
 
 hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a, int waves) {
     const int Wn = 2 * a.hw + 1;
+    /* at most 16 views and 7 draws per row (the configured workloads): the
+     * specialisation without the wide-view and many-draw paths */
+    if (a.V <= 16 && a.S <= 7) {
+        if (waves == 1 && Wn == 15) return launch_sweep_t<16, 4, 15, 1, 2, true>(s, a);
+        if (waves == 1 && Wn == 11) return launch_sweep_t<16, 2, 11, 1, 2, true>(s, a);
+        if (waves == 3 && Wn == 15) return launch_sweep_t<16, 4, 15, 2, 3, true>(s, a);
+        if (waves == 3 && Wn == 11) return launch_sweep_t<16, 2, 11, 2, 3, true>(s, a);
+    }
 #define PMX_SWEEP_NW(G, PF, WN)                                      \
     switch (waves) {                                                 \
     case 1: return launch_sweep_t<G, PF, WN, 1>(s, a);               \
