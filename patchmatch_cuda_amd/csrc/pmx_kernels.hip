@@ -861,14 +861,15 @@ __device__ __forceinline__ void combine_pass(int lane, int nt, int slices, const
 }
 
 /* One lane's share of a reference window, positions idx = lane + 64*k: the
- * loaded values as they arrive, and per position k bit k of `in` (inside
- * the image) and of `rd` (inside and read; else the value is 0).  Nothing
- * looks at a value before store_window, so a window prefetched a row ahead
- * is not waited for until it is stored. */
+ * loaded values as they arrive, and bit k of `in` (position k inside the
+ * image) and of `rd` (inside and read; else the value is 0).  Nothing looks
+ * at a value before store_window, so a window prefetched a row ahead is not
+ * waited for until it is stored. */
 template <int PF>
 struct WinVals {
+    using Mask = typename std::conditional<(PF <= 32), uint32_t, uint64_t>::type;
     float v[PF];
-    uint32_t in, rd;
+    Mask in, rd;
 };
 
 /* Loads the reference window of pixel (x, y) for one lane's share.
@@ -881,8 +882,9 @@ __device__ __forceinline__ void load_window(const PmxSweepArgs& a, int x, int y,
      * nothing load element 0, which is discarded) */
     const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
         (void*)a.ref, (short)0, a.ref_pitch * a.H * (int)sizeof(float), 0x00020000);
-    w.in = 0u;
-    w.rd = 0u;
+    using Mask = typename WinVals<PF>::Mask;
+    w.in = 0;
+    w.rd = 0;
 #pragma unroll
     for (int k = 0; k < PF; k++) {
         const int idx = lane + kLanes * k;
@@ -894,9 +896,72 @@ __device__ __forceinline__ void load_window(const PmxSweepArgs& a, int x, int y,
         const bool rd = in && (sx | sy) >= 0 && idx < Wn * Wn;
         const uint32_t off = rd ? (uint32_t)(sy * a.ref_pitch + sx) * 4u : 0u;
         w.v[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, off, 0, 0));
-        w.in |= (in ? 1u : 0u) << k;
-        w.rd |= (rd ? 1u : 0u) << k;
+        w.in |= (Mask)(in ? 1 : 0) << k;
+        w.rd |= (Mask)(rd ? 1 : 0) << k;
     }
+}
+
+/* The reference windows along one sweep column, one lane's share: position
+ * k of the window at sweep row r lies at varying coordinate r + dv[k] (y in
+ * a plain sweep, x in a rotated one) and at a fixed coordinate along the
+ * other axis, which carries the column's tile-index shift.  Its byte offset
+ * in the reference is r * rstride + off[k]; positions outside the image
+ * along the varying axis fall outside the buffer or are discarded by their
+ * flags, and a raw buffer load outside the buffer returns 0 without a
+ * fault.  fix_in / fix_rd hold the positions' flags as if the varying
+ * coordinate were inside.  For the small windows (a few positions per
+ * lane) whose per-lane constants fit the registers. */
+template <int PF>
+struct ColWindow {
+    static_assert(PF <= 8, "column window constants for small windows only");
+    int dv[PF], off[PF];
+    uint32_t fix_in, fix_rd;
+    int rstride, Lv;
+    __amdgpu_buffer_rsrc_t rsrc;
+};
+
+template <int PF>
+__device__ __forceinline__ ColWindow<PF> col_window(const PmxSweepArgs& a, bool rot, int c, int qf,
+                                                   int Wn, float inv_wn, int lane) {
+    ColWindow<PF> cw;
+    const int Lf = rot ? a.H : a.W;
+    cw.fix_in = 0u;
+    cw.fix_rd = 0u;
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        const int idx = lane + kLanes * k;
+        const int dyi = (int)(((float)idx + 0.5f) * inv_wn);
+        const int dxi = idx - dyi * Wn;
+        const int dv = (rot ? dxi : dyi) - a.hw;
+        const int bf = c + (rot ? dyi : dxi) - a.hw; /* fixed coordinate */
+        const bool inF = (unsigned)bf < (unsigned)Lf;
+        const bool rdF = inF && bf - qf >= 0 && idx < Wn * Wn;
+        cw.dv[k] = dv;
+        cw.off[k] = 4 * (rot ? (bf - qf) * a.ref_pitch + dv : dv * a.ref_pitch + (bf - qf));
+        cw.fix_in |= (inF ? 1u : 0u) << k;
+        cw.fix_rd |= (rdF ? 1u : 0u) << k;
+    }
+    cw.rstride = rot ? 4 : 4 * a.ref_pitch;
+    cw.Lv = rot ? a.W : a.H;
+    cw.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.ref, (short)0,
+                                                a.ref_pitch * a.H * (int)sizeof(float), 0x00020000);
+    return cw;
+}
+
+/* The window of sweep row r of the column. */
+template <int PF>
+__device__ __forceinline__ void load_col_window(const ColWindow<PF>& cw, int r, WinVals<PF>& w) {
+    const uint32_t rb = (uint32_t)(r * cw.rstride);
+    uint32_t inv = 0u;
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        const bool inV = (unsigned)(r + cw.dv[k]) < (unsigned)cw.Lv;
+        w.v[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                               cw.rsrc, rb + (uint32_t)cw.off[k], 0, 0));
+        inv |= (inV ? 1u : 0u) << k;
+    }
+    w.in = cw.fix_in & inv;
+    w.rd = cw.fix_rd & inv;
 }
 
 /* Writes a loaded window as R' = Rq - mean into LDS (0 outside the image).
@@ -1183,8 +1248,13 @@ __device__ __forceinline__ float proposal_draw(float depth, float spread, int x,
  * (the reference recomputes the same value).  Wave 0 alone writes the
  * column's outputs.
  */
-template <int G, int PF, int WN, int NW, bool H16, int NB = 0, bool SM = false>
+template <int G, int PF, int WN, int NW, bool H16, int NB = 0, bool SM = false, int ROT = -1,
+          int PRP = -1>
 __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c) {
+    /* sweep frame and proposal presence, fixed at compile time when ROT / PRP
+     * are 0 or 1 (-1: from the arguments) */
+    const bool rot = ROT >= 0 ? ROT == 1 : a.rotated != 0;
+    const bool prp = PRP >= 0 ? PRP == 1 : a.prop != nullptr;
     extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
     const int WW = 2 * Wn * Wn;                          /* window + zero rows */
@@ -1214,7 +1284,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     float* scol = a.spm + (size_t)c * a.spm_ld;
 
     /* generator of this column: row 0 (plain) or column 0 (rotated) */
-    pmx_rng_state* rp = a.rotated ? (pmx_rng_state*)((char*)a.rng + (size_t)c * a.rng_pitch)
+    pmx_rng_state* rp = rot ? (pmx_rng_state*)((char*)a.rng + (size_t)c * a.rng_pitch)
                                   : a.rng + c;
     Xw rng = xw_uniformize(xw_load(rp));
 
@@ -1240,7 +1310,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     const int r0 = a.down ? 0 : Hs_rows - 1;
     /* original-frame coordinates of sweep row r in this column */
     auto orig_xy = [&](int r, int& x, int& y) {
-        if (a.rotated) {
+        if (rot) {
             x = r;
             y = c;
         } else {
@@ -1256,8 +1326,8 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     }
     float d_former = dcol[r0];
 
-    const int qx_col = a.rotated ? 0 : max(0, a.hw - c);
-    const int qy_col = a.rotated ? max(0, a.hw - c) : 0;
+    const int qx_col = rot ? 0 : max(0, a.hw - c);
+    const int qy_col = rot ? max(0, a.hw - c) : 0;
     const bool need_d = qx_col != 0 || qy_col != 0;
 
     /* prefetch row r0 + step */
@@ -1266,18 +1336,24 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     float pf_beta = 0.0f;
     float2 pf_stat = make_float2(0.0f, 0.0f);
     WinVals<PF> pf_win;
+    constexpr int kColPF = PF <= 8 ? PF : 1; /* column constants for small windows only */
+    const ColWindow<kColPF> cwin =
+        col_window<kColPF>(a, rot, c, rot ? qy_col : qx_col, Wn, inv_wn, lane);
     auto prefetch = [&](int rr) {
         int x, y;
         orig_xy(rr, x, y);
         pf_cost = ccol[(size_t)rr * V + vl];
         pf_beta = scol[(size_t)rr * V + vl];
         pf_depth = dcol[rr];
-        if (a.prop) {
+        if (prp) {
             pf_prop = a.prop[(size_t)c * a.prop_ld + rr];
             pf_spread = a.prop_spread[(size_t)c * a.prop_ld + rr];
         }
         pf_stat = a.stats[(size_t)y * a.stats_pitch + x];
-        load_window<PF>(a, x, y, qx_col, qy_col, Wn, inv_wn, lane, pf_win);
+        if constexpr (PF <= 8)
+            load_col_window<PF>(cwin, rr, pf_win);
+        else
+            load_window<PF>(a, x, y, qx_col, qy_col, Wn, inv_wn, lane, pf_win);
     };
     if (Hs_rows > 1) prefetch(r);
     int buf = 0;
@@ -1308,13 +1384,13 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
          * wait for them as well (memory counters drain in order).  The
          * column-uniform ones move to scalar registers. */
         const float cst = pf_cost, beta = pf_beta;
-        const float d_cur = uniform_f(pf_depth), spread = uniform_f(pf_spread);
-        float d_prop = uniform_f(pf_prop);
+        const float d_cur = uniform_f(pf_depth), spread = prp ? uniform_f(pf_spread) : 0.0f;
+        float d_prop = prp ? uniform_f(pf_prop) : 0.0f;
         const float2 st = make_float2(uniform_f(pf_stat.x), uniform_f(pf_stat.y));
 
         int x, y;
         orig_xy(r, x, y);
-        if (d_prop > 0.0f && spread > 0.0f) d_prop = proposal_draw(d_prop, spread, x, y, a);
+        if (prp && d_prop > 0.0f && spread > 0.0f) d_prop = proposal_draw(d_prop, spread, x, y, a);
         PixelCtx pc;
         pixel_ctx(a, x, y, gl, pc);
         pc.mu = st.x;
@@ -1454,7 +1530,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         const bool in_sel = is_view && ((sel >> lane) & 1ull);
         const int srank = __popcll(sel & ((1ull << lane) - 1ull));
         /* a learned proposal is a fourth candidate on the same views */
-        const bool has_prop = d_prop > 0.0f;
+        const bool has_prop = prp && d_prop > 0.0f;
         const int per = (same ? 1 : 2) + (has_prop ? 1 : 0);
         if (in_sel) {
             int o = per * srank;
@@ -1565,12 +1641,13 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
  * (WPE = 3), which leaves 168 registers instead of 128; its taps go out in
  * batches of 6 rows, which fit them without spilling (8-row batches spill
  * 6 registers and take 1.3% longer per 2048x1536 run). */
-template <int G, int PF, int WN, int NW, bool H16, int WPE, bool SM = false>
+template <int G, int PF, int WN, int NW, bool H16, int WPE, bool SM = false, int ROT = -1,
+          int PRP = -1>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE)))
 void pmx_sweep_kernel(PmxSweepArgs a) {
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= a.Ws) return;
-    sweep_column<G, PF, WN, NW, H16, (NW == 2 && WPE == 3) ? 6 : 0, SM>(a, c);
+    sweep_column<G, PF, WN, NW, H16, (NW == 2 && WPE == 3) ? 6 : 0, SM, ROT, PRP>(a, c);
 }
 
 /*
@@ -1742,18 +1819,31 @@ hipError_t launch_volume_t(hipStream_t s, const PmxSweepArgs& a, int v0, int v1,
     return hipGetLastError();
 }
 
-template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW), bool SM = false>
+template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW), bool SM = false,
+          int ROT = -1, int PRP = -1>
 hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
     const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * 2 * 64 * NW +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     if (a.src_h16)
-        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, true, WPE, SM>), dim3(a.Ws),
+        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, true, WPE, SM, ROT, PRP>), dim3(a.Ws),
                            dim3(64 * NW), sh, s, a);
     else
-        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, false, WPE, SM>), dim3(a.Ws),
+        hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, false, WPE, SM, ROT, PRP>), dim3(a.Ws),
                            dim3(64 * NW), sh, s, a);
     return hipGetLastError();
+}
+
+/* The specialisation for at most 16 views and 7 draws per row, with the
+ * sweep frame and the proposal's presence fixed at compile time. */
+template <int G, int PF, int WN, int NW, int WPE>
+hipError_t launch_sweep_sm(hipStream_t s, const PmxSweepArgs& a) {
+    const bool p = a.prop != nullptr;
+    if (a.rotated)
+        return p ? launch_sweep_t<G, PF, WN, NW, WPE, true, 1, 1>(s, a)
+                 : launch_sweep_t<G, PF, WN, NW, WPE, true, 1, 0>(s, a);
+    return p ? launch_sweep_t<G, PF, WN, NW, WPE, true, 0, 1>(s, a)
+             : launch_sweep_t<G, PF, WN, NW, WPE, true, 0, 0>(s, a);
 }
 template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW)>
 hipError_t launch_sweep_batch_t(hipStream_t s, const PmxBatchLaunch& b) {
@@ -1861,10 +1951,10 @@ hipError_t pmx_launch_sweep(hipStream_t s, const PmxSweepArgs& a, int waves) {
     /* at most 16 views and 7 draws per row (the configured workloads): the
      * specialisation without the wide-view and many-draw paths */
     if (a.V <= 16 && a.S <= 7) {
-        if (waves == 1 && Wn == 15) return launch_sweep_t<16, 4, 15, 1, 2, true>(s, a);
-        if (waves == 1 && Wn == 11) return launch_sweep_t<16, 2, 11, 1, 2, true>(s, a);
-        if (waves == 3 && Wn == 15) return launch_sweep_t<16, 4, 15, 2, 3, true>(s, a);
-        if (waves == 3 && Wn == 11) return launch_sweep_t<16, 2, 11, 2, 3, true>(s, a);
+        if (waves == 1 && Wn == 15) return launch_sweep_sm<16, 4, 15, 1, 2>(s, a);
+        if (waves == 1 && Wn == 11) return launch_sweep_sm<16, 2, 11, 1, 2>(s, a);
+        if (waves == 3 && Wn == 15) return launch_sweep_sm<16, 4, 15, 2, 3>(s, a);
+        if (waves == 3 && Wn == 11) return launch_sweep_sm<16, 2, 11, 2, 3>(s, a);
     }
 #define PMX_SWEEP_NW(G, PF, WN)                                      \
     switch (waves) {                                                 \
