@@ -59,6 +59,23 @@ __device__ __forceinline__ float row_sum16(float v) {
     return v;
 }
 
+/* Sums of consecutive 16-lane rows, for values already equal across each
+ * row: with two rows per sum rows 1 and 3 receive rows 0+1 and 2+3; with
+ * four, row 3 receives the sum of all four (DPP row_bcast:15 and
+ * row_bcast:31, a row's lane 15 added to the next row).  Other rows keep
+ * their value.  FP32 addition of two terms commutes, so the sums equal an
+ * xor butterfly's bit for bit. */
+template <int ROWS>
+__device__ __forceinline__ float row_group_sum(float v) {
+    if (ROWS >= 2)
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                   0x142, 0xA, 0xF, false));
+    if (ROWS >= 4)
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                   0x143, 0x8, 0xF, false));
+    return v;
+}
+
 /* Inclusive prefix sum over each 16-lane row (FP32). */
 __device__ __forceinline__ float row_scan16f(float v) {
     v += dpp_f<kDppRowShr + 1>(v);
@@ -591,6 +608,34 @@ __device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const SB&
     return q;
 }
 
+/* A hypothesis' projection as four float4 in LDS (computed once, read by
+ * every lane group that sums its window). */
+__device__ __forceinline__ void store_hyp(float4* h, const Hyp& q) {
+    h[0] = make_float4(q.ux, q.uy, q.vx, q.vy);
+    h[1] = make_float4(q.cx, q.cy, q.fxc, q.fyc);
+    h[2] = make_float4(q.lox, q.hix, q.loy, q.hiy);
+    h[3] = make_float4(__uint_as_float(q.base), 0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ Hyp load_hyp(const float4* h) {
+    const float4 h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3];
+    Hyp q;
+    q.ux = h0.x;
+    q.uy = h0.y;
+    q.vx = h0.z;
+    q.vy = h0.w;
+    q.cx = h1.x;
+    q.cy = h1.y;
+    q.fxc = h1.z;
+    q.fyc = h1.w;
+    q.lox = h2.x;
+    q.hix = h2.y;
+    q.loy = h2.z;
+    q.hiy = h2.w;
+    q.base = __float_as_uint(h3.x);
+    return q;
+}
+
 /* Window rows [r0, r0 + nr) of one lane's window column; R is the static
  * trip count (0: dynamic).  Sums are taken around K: A = sum e,
  * B = sum e^2, C = sum e R' with e = I' - K.
@@ -761,12 +806,12 @@ __device__ __forceinline__ float ncc_finish(float A, float B, float C, float K,
  * sums to part[t * NW + slice] and the caller combines them after a
  * barrier (combine_pass).  Returns k.
  */
-template <int G, int WN, int NW, int NB = 0, bool INT = false, class SB>
+template <int G, int WN, int NW, int NB = 0, bool INT = false, bool PRE = false, class SB>
 __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __restrict__ Hm,
                                          const SB& sb, const float* __restrict__ win,
                                          int wave, int lane, int nt, const int* tview,
                                          const float* tdep, float* tout, float4* part,
-                                         const PixelCtx& pc) {
+                                         const PixelCtx& pc, const float4* hyps = nullptr) {
     constexpr int NG = kLanes / G;
     constexpr int NGT = NG * NW;
     /* tap rows in flight per lane: a whole column with one wave per
@@ -783,7 +828,7 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
     const bool active = t < nt;
     float A = 0.0f, B = 0.0f, C = 0.0f, K = 0.0f;
     if (active) {
-        const Hyp q = make_hyp(Hm, sb, tview[t], tdep[t], pc);
+        const Hyp q = PRE ? load_hyp(hyps + 4 * t) : make_hyp(Hm, sb, tview[t], tdep[t], pc);
         const int R = (Wn + k - 1) >> lk;
         const int r0 = max(sub * R, pc.dy_lo);
         const int r1 = min(min((sub + 1) * R, Wn), pc.dy_hi + 1);
@@ -817,10 +862,23 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
             accum_rows<0>(sb, q, Wn, dxf, dy0, wp, nr, valid, K, A, B, C);
         }
     }
+    const int kw = k < NG ? k : NG; /* groups of the hypothesis inside this wave */
+    /* the group that ends up holding the hypothesis' sums in this wave */
+    int last = 0;
     if constexpr (G == 16) {
         A = row_sum16(A);
         B = row_sum16(B);
         C = row_sum16(C);
+        if (kw == 2) {
+            A = row_group_sum<2>(A);
+            B = row_group_sum<2>(B);
+            C = row_group_sum<2>(C);
+        } else if (kw == 4) {
+            A = row_group_sum<4>(A);
+            B = row_group_sum<4>(B);
+            C = row_group_sum<4>(C);
+        }
+        last = kw - 1;
     } else {
 #pragma unroll
         for (int m = G / 2; m >= 1; m >>= 1) {
@@ -828,16 +886,20 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
             B += __shfl_xor(B, m, G);
             C += __shfl_xor(C, m, G);
         }
-    }
-    const int kw = k < NG ? k : NG; /* groups of the hypothesis inside this wave */
-    for (int m = G; m < G * kw; m <<= 1) {
-        A += __shfl_xor(A, m, kLanes);
-        B += __shfl_xor(B, m, kLanes);
-        C += __shfl_xor(C, m, kLanes);
+        for (int m = G; m < G * kw; m <<= 1) {
+            A += __shfl_xor(A, m, kLanes);
+            B += __shfl_xor(B, m, kLanes);
+            C += __shfl_xor(C, m, kLanes);
+        }
     }
     if (k <= NG) {
-        if (active && sub == 0 && gl == 0) tout[t] = ncc_finish(A, B, C, K, pc);
-    } else if (active && lane == 0) {
+        if (active && sub == last && gl == 0) {
+            if (PRE)
+                part[t * NW] = make_float4(A, B, C, K);
+            else
+                tout[t] = ncc_finish(A, B, C, K, pc);
+        }
+    } else if (active && (sub & (NG - 1)) == last && gl == 0) {
         part[t * NW + sub / NG] = make_float4(A, B, C, K);
     }
     return k;
@@ -1264,6 +1326,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     float* tres = winbuf + 2 * WW;                       /* task results: kTasks, then 64 */
     int* tview = (int*)(tres + kTasks + 64);             /* task views, kTasks */
     float* tdep = (float*)(tview + kTasks);              /* task depths, kTasks */
+    float4* hypb = (float4*)(tdep + kTasks);             /* projections of 64 hypotheses, 4 float4 each */
 
     constexpr int NG = kLanes / G;
     constexpr int NGT = NG * NW;
@@ -1365,6 +1428,35 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
 
     /* one phase of hypotheses tview/tdep[0..nt) -> tout[0..nt) */
     auto run_phase = [&](int nt, const float* win, float* tout, const PixelCtx& pc) {
+        if constexpr (SM) {
+            /* at most 3 x 7 hypotheses: each projection is computed once, by
+             * one lane, the passes leave their window sums in `part` (in the
+             * order a per-pass finish would combine them), and one lane per
+             * hypothesis forms its 1 - NCC */
+            for (int t = (int)threadIdx.x; t < nt; t += 64 * NW)
+                store_hyp(hypb + 4 * t, make_hyp(Hs, sb, tview[t], tdep[t], pc));
+            lds_barrier();
+            for (int t0 = 0; t0 < nt; t0 += NGT)
+                eval_pass<G, WN, NW, NB, true, true>(a, Hs, sb, win, wave, lane, min(NGT, nt - t0), nullptr,
+                                                     nullptr, nullptr, part + t0 * NW, pc, hypb + 4 * t0);
+            lds_barrier();
+            for (int t = (int)threadIdx.x; t < nt; t += 64 * NW) {
+                const int ntp = min(NGT, nt - (t & ~(NGT - 1)));
+                const int lk = __builtin_ctz((unsigned)NGT) -
+                               (ntp > 1 ? 32 - __builtin_clz((unsigned)(ntp - 1)) : 0);
+                const int slices = (1 << lk) > NG ? (1 << lk) / NG : 1;
+                float4 s = part[t * NW];
+                for (int w = 1; w < slices; w++) {
+                    const float4 q = part[t * NW + w];
+                    s.x += q.x;
+                    s.y += q.y;
+                    s.z += q.z;
+                }
+                tout[t] = ncc_finish(s.x, s.y, s.z, s.w, pc);
+            }
+            lds_barrier();
+            return;
+        }
         for (int t0 = 0; t0 < nt; t0 += NGT) {
             const int ntp = min(NGT, nt - t0);
             float4* pp = part + (pb & 1) * 64 * NW;
@@ -1823,7 +1915,7 @@ template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW), bo
           int ROT = -1, int PRP = -1>
 hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
-    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * 2 * 64 * NW +
+    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * (2 * 64 * NW + 4 * 64) +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     if (a.src_h16)
         hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, true, WPE, SM, ROT, PRP>), dim3(a.Ws),
