@@ -1467,6 +1467,16 @@ PmxBatchLaunch batch_launch(const pmx_batch* b, int k) {
     L.hw = b->engs[0]->p.half_window;
     L.max_V = b->max_V;
     L.h16 = b->engs[0]->storage == PMX_STORE_F16;
+    L.rot = kSweepDir[k % 4][1];
+    int max_S = 1, with_prop = 0;
+    for (const pmx_engine* e : b->engs) {
+        max_S = std::max(max_S, e->p.num_samples);
+        with_prop += e->has_prop ? 1 : 0;
+    }
+    L.prp = with_prop > 0 ? 1 : 0;
+    /* the small-view kernels fix the proposal's presence for all views */
+    const bool uniform_prop = with_prop == 0 || with_prop == n;
+    L.sm = (b->max_V <= 16 && max_S <= 7 && uniform_prop) ? 1 : 0;
     return L;
 }
 

@@ -86,6 +86,9 @@ struct PmxBatchLaunch {
     int grid;                /* resident workgroups */
     int waves;               /* launch shape: 1 or 2 waves per column strip, or 3 */
     int hw, max_V, h16;
+    int rot;                 /* the step's sweep frame (1: rotated) */
+    int prp;                 /* every view carries a proposal */
+    int sm;                  /* every view has at most 16 sources and 7 draws per row */
 };
 
 struct pmx_engine {

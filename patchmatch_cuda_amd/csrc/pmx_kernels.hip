@@ -1274,6 +1274,9 @@ __device__ __forceinline__ float trans_g(float l) {
 /* Hypotheses of one row's phase B: up to 3 per sampled view (former,
  * random, proposal), at most 64 sampled views. */
 constexpr int kTasks = 3 * 64;
+/* Hypotheses of one phase in the small-view kernels: at most 3 per sampled
+ * view in phase B (at most 7 draws), at most 16 in phase C. */
+constexpr int kHypMax = 24;
 
 /* lowbias32 integer hash (C. Wellons, public domain constants). */
 __device__ __forceinline__ uint32_t mix32(uint32_t v) {
@@ -1326,7 +1329,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     float* tres = winbuf + 2 * WW;                       /* task results: kTasks, then 64 */
     int* tview = (int*)(tres + kTasks + 64);             /* task views, kTasks */
     float* tdep = (float*)(tview + kTasks);              /* task depths, kTasks */
-    float4* hypb = (float4*)(tdep + kTasks);             /* projections of 64 hypotheses, 4 float4 each */
+    float4* hypb = (float4*)(tdep + kTasks);             /* kHypMax projections, 4 float4 each */
 
     constexpr int NG = kLanes / G;
     constexpr int NGT = NG * NW;
@@ -1765,7 +1768,8 @@ void pmx_sweep_kernel(PmxSweepArgs a) {
  * 5 with FP32 (the fastest on the 16-view 10-source workload; 4 to 16
  * measured).
  */
-template <int G, int PF, int WN, int NW, bool H16, int WPE>
+template <int G, int PF, int WN, int NW, bool H16, int WPE, bool SM = false, int ROT = -1,
+          int PRP = -1>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE)))
 void pmx_sweep_batch_kernel(const PmxSweepArgs* __restrict__ tab, const int* __restrict__ qstart,
                             const int* __restrict__ qcol, int n, unsigned* __restrict__ queue) {
@@ -1782,7 +1786,7 @@ void pmx_sweep_batch_kernel(const PmxSweepArgs* __restrict__ tab, const int* __r
         if (j >= total) break;
         int v = 0;
         while (st[v + 1] <= j) v++;
-        sweep_column<G, PF, WN, NW, H16, NB>(tab[v], qcol[x * n + v] + (j - st[v]));
+        sweep_column<G, PF, WN, NW, H16, NB, SM, ROT, PRP>(tab[v], qcol[x * n + v] + (j - st[v]));
         __syncthreads();
     }
 }
@@ -1915,7 +1919,7 @@ template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW), bo
           int ROT = -1, int PRP = -1>
 hipError_t launch_sweep_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
-    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * (2 * 64 * NW + 4 * 64) +
+    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * (2 * 64 * NW + (SM ? 4 * kHypMax : 0)) +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     if (a.src_h16)
         hipLaunchKernelGGL((pmx_sweep_kernel<G, PF, WN, NW, true, WPE, SM, ROT, PRP>), dim3(a.Ws),
@@ -1937,30 +1941,48 @@ hipError_t launch_sweep_sm(hipStream_t s, const PmxSweepArgs& a) {
     return p ? launch_sweep_t<G, PF, WN, NW, WPE, true, 0, 1>(s, a)
              : launch_sweep_t<G, PF, WN, NW, WPE, true, 0, 0>(s, a);
 }
-template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW)>
+template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW), bool SM = false,
+          int ROT = -1, int PRP = -1>
 hipError_t launch_sweep_batch_t(hipStream_t s, const PmxBatchLaunch& b) {
     const int Wn = 2 * b.hw + 1;
-    const size_t sh = sizeof(double) * b.max_V * 18 + sizeof(float4) * 2 * 64 * NW +
+    const size_t sh = sizeof(double) * b.max_V * 18 + sizeof(float4) * (2 * 64 * NW + (SM ? 4 * kHypMax : 0)) +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     if (b.h16)
-        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, true, WPE>), dim3(b.grid),
+        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, true, WPE, SM, ROT, PRP>), dim3(b.grid),
                            dim3(64 * NW), sh, s, b.tab, b.qstart, b.qcol, b.n, b.queue);
     else
-        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, false, WPE>), dim3(b.grid),
+        hipLaunchKernelGGL((pmx_sweep_batch_kernel<G, PF, WN, NW, false, WPE, SM, ROT, PRP>), dim3(b.grid),
                            dim3(64 * NW), sh, s, b.tab, b.qstart, b.qcol, b.n, b.queue);
     return hipGetLastError();
 }
 
-template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW)>
+template <int G, int PF, int WN, int NW, int WPE = (2 * NW > 8 ? 8 : 2 * NW), bool SM = false,
+          int ROT = -1, int PRP = -1>
 hipError_t occupancy_batch_t(const PmxBatchLaunch& b, int* per_cu) {
     const int Wn = 2 * b.hw + 1;
-    const size_t sh = sizeof(double) * b.max_V * 18 + sizeof(float4) * 2 * 64 * NW +
+    const size_t sh = sizeof(double) * b.max_V * 18 + sizeof(float4) * (2 * 64 * NW + (SM ? 4 * kHypMax : 0)) +
                       sizeof(float) * (4 * Wn * Wn + 3 * kTasks + 64 + 64);
     if (b.h16)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, true, WPE>, 64 * NW, sh);
+            per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, true, WPE, SM, ROT, PRP>, 64 * NW, sh);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, false, WPE>, 64 * NW, sh);
+        per_cu, pmx_sweep_batch_kernel<G, PF, WN, NW, false, WPE, SM, ROT, PRP>, 64 * NW, sh);
+}
+
+/* Batched sweeps of views with at most 16 sources and 7 draws per row: the
+ * sweep frame and the proposal's presence fixed at compile time. */
+template <int G, int PF, int WN, int NW, int WPE>
+hipError_t batch_sm(hipStream_t s, const PmxBatchLaunch& b, int* per_cu) {
+#define PMX_BATCH_SM(R, P)                                                              \
+    return per_cu ? occupancy_batch_t<G, PF, WN, NW, WPE, true, R, P>(b, per_cu)       \
+                  : launch_sweep_batch_t<G, PF, WN, NW, WPE, true, R, P>(s, b);
+    if (b.rot) {
+        if (b.prp) { PMX_BATCH_SM(1, 1) }
+        PMX_BATCH_SM(1, 0)
+    }
+    if (b.prp) { PMX_BATCH_SM(0, 1) }
+    PMX_BATCH_SM(0, 0)
+#undef PMX_BATCH_SM
 }
 } // namespace
 
@@ -1971,6 +1993,12 @@ hipError_t occupancy_batch_t(const PmxBatchLaunch& b, int* per_cu) {
  * launching. */
 hipError_t pmx_launch_sweep_batch(hipStream_t s, const PmxBatchLaunch& b, int* per_cu) {
     const int Wn = 2 * b.hw + 1;
+    if (b.sm) {
+        if (b.waves == 3 && Wn == 15) return batch_sm<16, 4, 15, 1, 3>(s, b, per_cu);
+        if (b.waves == 3 && Wn == 11) return batch_sm<16, 2, 11, 1, 3>(s, b, per_cu);
+        if (b.waves == 1 && Wn == 15) return batch_sm<16, 4, 15, 1, 2>(s, b, per_cu);
+        if (b.waves == 1 && Wn == 11) return batch_sm<16, 2, 11, 1, 2>(s, b, per_cu);
+    }
     if (b.waves == 3 && Wn == 15)
         return per_cu ? occupancy_batch_t<16, 4, 15, 1, 3>(b, per_cu)
                       : launch_sweep_batch_t<16, 4, 15, 1, 3>(s, b);
