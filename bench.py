@@ -118,7 +118,7 @@ def crop_problem(sc, sample):
 
 
 def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, parity_crop, steps, warmup, tau,
-                seeding):
+                seeding, prop_sweeps):
     """Learned-proposal PatchMatch against plain PatchMatch.
 
     Sweeps needed: on the view the accuracy oracle covered (sample None: the
@@ -142,6 +142,7 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, parity_crop, steps,
     net = proposal.Net(ctx, 1, H0, W0, rp.depth_near, rp.depth_far)
     net.load(weights)
     eng.set_proposal_seeding(seeding)
+    eng.set_proposal_sweeps(prop_sweeps)
     eng.prepare()
     prop, spread = proposal_planes(ctx, eng, net, tau)
     eng.set_proposal(None, 0)
@@ -152,6 +153,7 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, parity_crop, steps,
         (x0, y0, w, h), (ref, srcs, cams, gt) = crop_problem(sc, sample)
         seng = capi.Engine(ctx, rp, ref, list(srcs), cams)
         seng.set_proposal_seeding(seeding)
+        seng.set_proposal_sweeps(prop_sweeps)
         spd = capi.DeviceArray(ctx, (h, w), np.float32).upload(
             np.ascontiguousarray(prop[y0:y0 + h, x0:x0 + w]))
         ssd = capi.DeviceArray(ctx, (h, w), np.float32).upload(
@@ -199,6 +201,7 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, parity_crop, steps,
     sc_ = np.ascontiguousarray(spread[y0:y0 + h, x0:x0 + w])
     ceng = capi.Engine(ctx, rp, ref, list(srcs), cams)
     ceng.set_proposal_seeding(seeding)
+    ceng.set_proposal_sweeps(prop_sweeps)
     pd = capi.DeviceArray(ctx, (h, w), np.float32).upload(pc)
     sd = capi.DeviceArray(ctx, (h, w), np.float32).upload(sc_)
     ceng.set_num_iterations(k)
@@ -213,6 +216,7 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, parity_crop, steps,
                    f(rp.depth_far), f(rp.spm_alpha), O.inv_k4(cams[0, :9]),
                    f(rp.orientation_x), f(rp.orientation_z))
     pb.set_proposal(pc.astype(np.float64), sc_.astype(np.float64))
+    pb.set_proposal_sweeps(prop_sweeps)
     do, _, _ = pb.run(d0.astype(np.float64), st0.copy(), rp.num_samples, k, record_ids=False)
     parity = float(np.mean(np.abs(dg - do) <= 0.01 * np.abs(do)))
     log(f"learned on the {w}x{h} crop, {k} iteration(s): {parity:.4f} of depths within 1% of the "
@@ -265,6 +269,7 @@ def learned_leg(ctx, eng, sc, rp, weights, d_oracle, sample, parity_crop, steps,
                              f"proposal planes"},
         "tau": tau,
         "proposal_seeds_depth": bool(seeding),
+        "proposal_sweeps": prop_sweeps,
         "weights": os.path.relpath(weights, ROOT),
     }
 
@@ -526,6 +531,9 @@ def main():
                     help="proposal network weights (default: the packaged file)")
     ap.add_argument("--tau", type=float, default=0.2,
                     help="confidence threshold of the learned proposal")
+    ap.add_argument("--proposal-sweeps", type=int, default=4,
+                    help="sweeps of a learned run that get the proposal (0: all; 4: the "
+                         "first iteration)")
     ap.add_argument("--proposal-seeding", type=int, default=0,
                     help="1: the learned proposal also seeds the initial depth")
     ap.add_argument("--no-learned", action="store_true")
@@ -653,7 +661,7 @@ def main():
         lr = learned_leg(ctx, eng, sc, rp, weights, d_oracle,
                          None if args.cpu_sample == "full" else parse_sample(args.cpu_sample),
                          parse_sample(args.parity_crop), args.steps, args.warmup, args.tau,
-                         bool(args.proposal_seeding))
+                         bool(args.proposal_seeding), args.proposal_sweeps)
         # both at the iteration count each needs to reach the accuracy bar
         need = lr["iterations_to_99pct_of_converged_oracle"]
         lr["speedup_vs_plain"] = lr["speedup_vs_plain_at_equal_accuracy"]

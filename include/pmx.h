@@ -325,6 +325,11 @@ int pmx_engine_set_proposal(pmx_engine* eng, const float* depth_dev, const float
 int pmx_engine_volume_views(pmx_engine* eng, int* view0, int* view1);
 /* Whether a proposal also seeds the depth (default 1). */
 int pmx_engine_set_proposal_seeding(pmx_engine* eng, int seed_depth);
+/* The proposal hypothesis joins only the first `sweeps` sweeps of a run
+ * (counted from the last prepare; 0, the default: every sweep).  No
+ * reference counterpart: 4 limits it to the first iteration, where the
+ * reference's depths are still its random initial ones. */
+int pmx_engine_set_proposal_sweeps(pmx_engine* eng, int sweeps);
 /* numOfIterations after construction (the proposal needs fewer). */
 int pmx_engine_set_num_iterations(pmx_engine* eng, int iterations);
 

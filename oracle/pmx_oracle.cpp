@@ -111,6 +111,7 @@ struct pmxo_problem {
      * frame (H*W); empty = none */
     std::vector<double> prop, prop_t, spread, spread_t;
     mutable int sweep_id = 0; /* sweeps since the proposal was set */
+    int prop_sweeps = 0;      /* sweeps that get the proposal (0: all) */
     mutable std::atomic<uint64_t> ncc_evals{0};
     mutable std::atomic<uint64_t> samples{0};
 };
@@ -624,7 +625,9 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
              * sampled views, compared last */
             const std::vector<double>& pv = rotated ? pb->prop_t : pb->prop;
             const std::vector<double>& sv = rotated ? pb->spread_t : pb->spread;
-            double d_prop = pv.empty() ? 0.0 : pv[(size_t)r * Ws + c];
+            const bool use_prop =
+                !pv.empty() && (pb->prop_sweeps <= 0 || pb->sweep_id < pb->prop_sweeps);
+            double d_prop = use_prop ? pv[(size_t)r * Ws + c] : 0.0;
             if (d_prop > 0.0 && sv[(size_t)r * Ws + c] > 0.0)
                 d_prop = proposal_draw((float)d_prop, (float)sv[(size_t)r * Ws + c],
                                        rotated ? r : c, rotated ? c : r, p, pb->sweep_id);
@@ -748,6 +751,8 @@ void pmxo_problem_set_proposal(pmxo_problem* pb, const double* prop, const doubl
 }
 
 void pmxo_problem_set_sweep_id(pmxo_problem* pb, int sweep_id) { pb->sweep_id = sweep_id; }
+
+void pmxo_problem_set_proposal_sweeps(pmxo_problem* pb, int sweeps) { pb->prop_sweeps = sweeps; }
 
 double pmxo_proposal_draw(float depth, float spread, int x, int y, int W, double near_d,
                           double far_d, int sweep_id) {

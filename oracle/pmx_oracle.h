@@ -124,6 +124,9 @@ void   pmxo_hmm_posteriors(const double* costs, int L, double alpha, double* out
  * every pmxo_sweep advances it. */
 void   pmxo_problem_set_proposal(pmxo_problem* pb, const double* prop, const double* spread);
 void   pmxo_problem_set_sweep_id(pmxo_problem* pb, int sweep_id);
+/* The proposal joins only the sweeps with count below `sweeps` (0: all), as
+ * pmx_engine_set_proposal_sweeps of the engine. */
+void   pmxo_problem_set_proposal_sweeps(pmxo_problem* pb, int sweeps);
 /* The proposal hypothesis of pixel (x, y) of a W-wide image in sweep k. */
 double pmxo_proposal_draw(float depth, float spread, int x, int y, int W, double near_d,
                           double far_d, int sweep_id);

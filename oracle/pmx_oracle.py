@@ -80,6 +80,7 @@ def lib():
             "pmxo_run": (None, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
             "pmxo_counters": (None, [_vp, _vp, _vp]),
             "pmxo_problem_set_proposal": (None, [_vp, _vp, _vp]),
+            "pmxo_problem_set_proposal_sweeps": (None, [_vp, ctypes.c_int]),
             "pmxo_problem_set_sweep_id": (None, [_vp, ctypes.c_int]),
             "pmxo_proposal_draw": (ctypes.c_double, [ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                                      ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -298,6 +299,10 @@ class Problem:
 
     def set_sweep_id(self, k):
         lib().pmxo_problem_set_sweep_id(self._h, int(k))
+
+    def set_proposal_sweeps(self, n):
+        """The proposal joins only the first n sweeps (0: all)."""
+        lib().pmxo_problem_set_proposal_sweeps(self._h, int(n))
 
     def counters(self):
         a = ctypes.c_uint64()

@@ -132,6 +132,7 @@ SIGNATURES = {
     "pmx_engine_volume_views": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "pmx_engine_set_num_iterations": [_vp, _i],
     "pmx_engine_set_proposal_seeding": [_vp, _i],
+    "pmx_engine_set_proposal_sweeps": [_vp, _i],
     "pmx_engine_propose": [_vp, _vp, _i, _i, _f],
     "pmx_engine_set_storage": [_vp, _i],
     "pmx_engine_footprint": [ctypes.POINTER(Params), _i, _i, _i, _vp, _vp, _i, _i,
@@ -458,6 +459,10 @@ class Engine:
 
     def set_proposal_seeding(self, on=True):
         self._c(lib().pmx_engine_set_proposal_seeding(self.h, int(bool(on))))
+
+    def set_proposal_sweeps(self, n):
+        """The proposal joins only the first n sweeps of a run (0: all)."""
+        self._c(lib().pmx_engine_set_proposal_sweeps(self.h, int(n)))
 
     def set_num_iterations(self, n):
         self._c(lib().pmx_engine_set_num_iterations(self.h, n))

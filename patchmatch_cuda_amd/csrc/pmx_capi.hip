@@ -227,7 +227,7 @@ PmxSweepArgs sweep_args(pmx_engine* e, int rotated) {
     a.prop_spread = nullptr;
     a.prop_ld = 0;
     a.sweep_id = e->sweep_count;
-    if (e->has_prop) {
+    if (e->has_prop && (e->prop_sweeps <= 0 || e->sweep_count < e->prop_sweeps)) {
         a.prop = rotated ? e->prop_rm : e->prop_cm;
         a.prop_spread = rotated ? e->spread_rm : e->spread_cm;
         a.prop_ld = rotated ? e->prop_rm_ld : e->prop_cm_ld;
@@ -1031,6 +1031,13 @@ int pmx_engine_set_proposal_seeding(pmx_engine* e, int seed_depth) {
     return PMX_OK;
 }
 
+int pmx_engine_set_proposal_sweeps(pmx_engine* e, int sweeps) {
+    PMXCHK(check_engine(e));
+    if (sweeps < 0) return fail(e->ctx, PMX_ERR_ARG, "proposal sweeps must be >= 0 (got %d)", sweeps);
+    e->prop_sweeps = sweeps;
+    return PMX_OK;
+}
+
 int pmx_engine_set_num_iterations(pmx_engine* e, int iterations) {
     PMXCHK(check_engine(e));
     if (iterations < 1) return fail(e->ctx, PMX_ERR_ARG, "iterations must be at least 1");
@@ -1395,6 +1402,7 @@ std::vector<long long> batch_signature(const pmx_batch* b) {
         s.push_back(e->p.num_iterations);
         s.push_back(e->storage);
         s.push_back(e->has_prop ? 1 : 0);
+        s.push_back(e->prop_sweeps);
     }
     return s;
 }
@@ -1426,6 +1434,10 @@ int batch_build_tables(pmx_batch* b) {
             a.S = it == 0 ? 1 : e->p.num_samples; /* src/patchMatch.cu:328-331 */
             a.ids = ids ? e->used_ids : nullptr;
             a.sweep_id = k;
+            if (e->prop_sweeps > 0 && k >= e->prop_sweeps) {
+                a.prop = nullptr;
+                a.prop_spread = nullptr;
+            }
             tab[(size_t)k * n + j] = a;
         }
         for (int x = 0; x < 8; x++) {
@@ -1471,7 +1483,7 @@ PmxBatchLaunch batch_launch(const pmx_batch* b, int k) {
     int max_S = 1, with_prop = 0;
     for (const pmx_engine* e : b->engs) {
         max_S = std::max(max_S, e->p.num_samples);
-        with_prop += e->has_prop ? 1 : 0;
+        with_prop += (e->has_prop && (e->prop_sweeps <= 0 || k < e->prop_sweeps)) ? 1 : 0;
     }
     L.prp = with_prop > 0 ? 1 : 0;
     /* the small-view kernels fix the proposal's presence for all views */

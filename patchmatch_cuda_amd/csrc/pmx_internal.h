@@ -144,6 +144,7 @@ struct pmx_engine {
     float* spread_cm = nullptr;
     bool has_prop = false;
     bool prop_seeds = true;     /* the proposal seeds the depth */
+    int prop_sweeps = 0;        /* sweeps of a run that get the proposal (0: all) */
     int sweep_count = 0;        /* sweeps since the last prepare */
 
     unsigned long long* counters = nullptr;

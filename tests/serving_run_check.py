@@ -76,6 +76,7 @@ def main():
             prop = spread = None
             if learned:
                 e.set_proposal_seeding(False)
+                e.set_proposal_sweeps(srv.proposal_sweeps)
                 W0, H0 = proposal.low_res(e.W, e.H)
                 net = proposal.Net(ctx, 1, H0, W0, rp.depth_near, rp.depth_far)
                 net.load(weights)
@@ -93,6 +94,7 @@ def main():
                 pb = oracle_problem(r16, s16, e.homographies(), sc.cams[0], rp)
                 if learned:
                     pb.set_proposal(prop.astype(np.float64), spread.astype(np.float64))
+                    pb.set_proposal_sweeps(srv.proposal_sweeps)
                 t = time.perf_counter()
                 do, _, _ = pb.run(d0.astype(np.float64), st0.copy(), rp.num_samples,
                                   rp.num_iterations, record_ids=False)
@@ -109,6 +111,7 @@ def main():
     for c in ctxs:
         c.close()
     out = {"mode": a.mode, "rate_req_s": a.rate, "duration_s": a.duration, "seed": a.seed,
+           "proposal_sweeps": srv.proposal_sweeps if learned else None,
            "requests": len(stream), "budget_gib": a.budget_gib, "served": res,
            "checks": checks, "checks_passed": bool(ok)}
     print(json.dumps(out), flush=True)

@@ -345,6 +345,7 @@ def test_c5_serving_stream_learned(gpu_ctx):
         e.set_storage(capi.STORE_F16)
         e.set_sweep_waves(1)
         e.set_proposal_seeding(False)
+        e.set_proposal_sweeps(srv.proposal_sweeps)
         W0, H0 = proposal.low_res(e.W, e.H)
         net = proposal.Net(gpu_ctx, 1, H0, W0, rp.depth_near, rp.depth_far)
         net.load(proposal.DEFAULT_WEIGHTS)
@@ -358,6 +359,7 @@ def test_c5_serving_stream_learned(gpu_ctx):
             r16, s16 = e.images()
             pb = oracle_problem(r16, s16, e.homographies(), sc.cams[0], rp)
             pb.set_proposal(prop.astype(np.float64), spread.astype(np.float64))
+            pb.set_proposal_sweeps(srv.proposal_sweeps)
             do, _, _ = pb.run(d0.astype(np.float64), st0.copy(), rp.num_samples,
                               rp.num_iterations, record_ids=False)
             agree = within(served, do)

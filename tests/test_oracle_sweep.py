@@ -141,3 +141,31 @@ def test_proposal_draw_restated():
     hs = np.array(hs)
     assert hs.min() >= -1.0 and hs.max() < 1.0
     assert abs(hs.mean()) < 0.1 and np.histogram(hs, 4, (-1, 1))[0].min() > len(hs) / 8
+
+
+def test_proposal_sweeps_limit(setup):
+    """pmxo_problem_set_proposal_sweeps: a limit covering every sweep of the
+    run (or 0) leaves the run equal, bit for bit, to one with the proposal in
+    every sweep; a limit of one iteration's 4 sweeps changes a 3-iteration
+    run but not a 1-iteration one."""
+    sc, rp, pb, st, d0 = setup
+    prop = (sc.gt_depth * 1.02).astype(np.float64)
+    spread = np.full(prop.shape, 0.02)
+
+    def run(limit, iters):
+        pb.set_proposal(prop, spread)
+        pb.set_proposal_sweeps(limit)
+        try:
+            return pb.run(d0, st.copy(), rp.num_samples, iters)[0]
+        finally:
+            pb.set_proposal_sweeps(0)
+            pb.set_proposal(None)
+
+    every = run(0, 3)
+    assert np.array_equal(run(12, 3), every)
+    assert np.array_equal(run(1000, 3), every)
+    assert np.array_equal(run(4, 1), run(0, 1))
+    first = run(4, 3)
+    assert not np.array_equal(first, every)
+    plain = pb.run(d0, st.copy(), rp.num_samples, 3)[0]
+    assert not np.array_equal(first, plain)

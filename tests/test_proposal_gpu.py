@@ -186,3 +186,45 @@ def test_ground_truth_proposal_improves_depth(c1, gpu_ctx):
     eng.set_num_iterations(rp.num_iterations)
     _say(f"  within 1% of ground truth: plain {plain:.4f}, proposal (1 iteration) {prop:.4f}")
     assert prop > plain
+
+
+def test_proposal_sweeps_limit(c1, gpu_ctx):
+    """A proposal limited to the first 4 sweeps (pmx_engine_set_proposal_sweeps)
+    equals the schedule run stage by stage with the proposal set for the
+    first iteration and cleared after it, bit for bit; 0 restores every
+    sweep."""
+    spec, rp, sc, eng = c1
+    _say("case: proposal in the first iteration only (C1)")
+    g = _plane(gpu_ctx, (sc.gt_depth * 1.02).astype(np.float32))
+    sp = _plane(gpu_ctx, np.full((eng.H, eng.W), 0.01, np.float32))
+    eng.set_proposal_seeding(False)
+    try:
+        eng.set_proposal(g.ptr, eng.W * 4, sp.ptr)
+        eng.set_proposal_sweeps(4)
+        eng.prepare()
+        eng.run()
+        d_a, c_a, i_a, r_a = eng.depth(), eng.costs(), eng.used_ids(), eng.rng()
+        eng.set_proposal_sweeps(0)
+        eng.prepare()
+        eng.cost_all_views(rotated=True)
+        last = rp.num_iterations - 1
+        for it in range(rp.num_iterations):
+            if it == 1:
+                eng.set_proposal(None, 0)
+            S = 1 if it == 0 else rp.num_samples
+            for j, (down, rot) in enumerate(((1, 1), (1, 0), (0, 1), (0, 0))):
+                eng.sweep(down, rot, S, record_ids=(it == last and j == 3))
+        d_b, c_b, i_b, r_b = eng.depth(), eng.costs(), eng.used_ids(), eng.rng()
+        assert np.array_equal(d_a, d_b)
+        assert np.array_equal(c_a, c_b)
+        assert np.array_equal(i_a, i_b)
+        assert np.array_equal(r_a.view(np.uint32), r_b.view(np.uint32))
+        # and the limit changes the run: the proposal is used in later sweeps by default
+        eng.set_proposal(g.ptr, eng.W * 4, sp.ptr)
+        eng.prepare()
+        eng.run()
+        assert not np.array_equal(eng.depth(), d_a)
+    finally:
+        eng.set_proposal(None, 0)
+        eng.set_proposal_sweeps(0)
+        eng.set_proposal_seeding(True)
