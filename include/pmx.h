@@ -182,8 +182,9 @@ int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format,
                       const int* src_pitch_bytes, const pmx_camera* ref_cam,
                       const pmx_camera* src_cams, pmx_engine** out);
 int pmx_engine_destroy(pmx_engine* eng);
-/* Device bytes an engine of these sizes holds (pmx_engine_create, plus the
- * FP16 block of PMX_STORE_F16 and the planes of a proposal), without
+/* Device bytes an engine of these sizes holds (pmx_engine_create, the
+ * texel block of the storage (FP32 pairs or FP16) and the planes of a
+ * proposal), without
  * touching a device: the admission measure of a caller that keeps many
  * views resident within an HBM budget.  No reference counterpart (the
  * reference sizes its buffers inline, src/patchMatch.cu:51-206). */
@@ -248,11 +249,14 @@ int pmx_engine_reset_stats(pmx_engine* eng);
 
 /* Texel storage of the pre-blurred images the NCC reads (PMX_STORE_*).  No
  * reference counterpart: the reference samples uchar textures normalised to
- * float (src/patchMatch.cu:27-28, :192-200).  PMX_STORE_F16 rounds the
- * blurred sources and reference to the nearest FP16 value (ties to even)
- * and halves the bytes of every bilinear tap; homographies stay FP64 and
- * window sums FP32.  pmx_engine_get_images returns the rounded values.
- * Takes effect at the next pmx_engine_prepare, which it requires. */
+ * float (src/patchMatch.cu:27-28, :192-200).  PMX_STORE_F32 (the default)
+ * keeps each blurred source texel with its right neighbour's difference
+ * (8 bytes per texel, so a bilinear row blend is one FMA).  PMX_STORE_F16
+ * rounds the blurred sources and reference to the nearest FP16 value (ties
+ * to even), stores 2 bytes per texel and frees the FP32 pairs; homographies
+ * stay FP64 and window sums FP32.  pmx_engine_get_images returns the
+ * rounded values.  Takes effect at the next pmx_engine_prepare, which it
+ * requires. */
 int pmx_engine_set_storage(pmx_engine* eng, int storage);
 int pmx_engine_get_storage(pmx_engine* eng, int* storage);
 

@@ -188,8 +188,8 @@ PmxSweepArgs sweep_args(pmx_engine* e, int rotated) {
         a.src_bytes = (long)sizeof(uint16_t) * e->src_layer * e->V;
         a.src_h16 = 1;
     } else {
-        a.src_block = e->src;
-        a.src_bytes = (long)sizeof(float) * e->src_layer * e->V;
+        a.src_block = e->src2;
+        a.src_bytes = (long)sizeof(float2) * e->src_layer * e->V;
         a.src_h16 = 0;
     }
     a.src_layer = e->src_layer;
@@ -617,7 +617,8 @@ static void engine_free(pmx_engine* e) {
     void* bufs[] = {e->src_raw, e->src,     e->ref_raw,  e->ref,     e->sum_I,    e->sum_II,
                     e->stats,   e->Hm,      e->depth_rm, e->depth_cm, e->cost_rm, e->cost_cm,
                     e->spm,     e->rng,     e->used_ids, e->out_depth, e->out_cost, e->counters,
-                    e->prop_rm, e->prop_cm, e->spread_rm, e->spread_cm, e->src16};
+                    e->prop_rm, e->prop_cm, e->spread_rm, e->spread_cm, e->src16,
+                    e->src2};
     for (void* b : bufs) dev_free(b);
     for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
     delete e;
@@ -730,8 +731,16 @@ int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format, 
                     "source block of %zu bytes exceeds the 2 GiB addressable by one engine",
                     src_bytes);
     }
+    /* tap offsets are formed by a signed 24-bit multiply of the row index
+     * and the row pitch in bytes (8 per texel pair) */
+    if (e->src_pitch >= (1 << 20) || maxH >= (1 << 22)) {
+        engine_free(e);
+        return fail(ctx, PMX_ERR_ARG, "source images of %d x %d: at most 1048508 x 4194303 supported",
+                    maxW, maxH);
+    }
     ALLOC(e->src_raw, src_bytes);
     ALLOC(e->src, src_bytes);
+    ALLOC(e->src2, 2 * src_bytes); /* the default storage's texel pairs (< 4 GiB) */
     HIPCHK_E(hipMemsetAsync(e->src_raw, 0, src_bytes, s));
     e->ref_pitch = round_up(W, 64);
     const size_t plane = sizeof(float) * (size_t)e->ref_pitch * H;
@@ -872,7 +881,10 @@ int pmx_engine_footprint(const pmx_params* params, int ref_width, int ref_height
     t += g((size_t)round_up(ref_width, 64) * H * (size_t)params->num_samples);
     t += g(sizeof(float) * ref_pitch * H * V);                        /* out_cost */
     t += g(sizeof(unsigned long long) * 4);
-    if (storage == PMX_STORE_F16) t += g(sizeof(uint16_t) * src_layer * V);
+    if (storage == PMX_STORE_F16)
+        t += g(sizeof(uint16_t) * src_layer * V);
+    else
+        t += g(sizeof(float2) * src_layer * V);
     if (with_proposal) {
         t += 2 * g(sizeof(float) * (size_t)round_up(ref_width, 64) * H);
         t += 2 * g(sizeof(float) * (size_t)round_up(ref_height, 64) * W);
@@ -916,6 +928,10 @@ static int prepare_impl(pmx_engine* e, Timer* tm) {
     if (e->storage == PMX_STORE_F16) {
         HIPCHK(ctx, pmx_launch_quantize_f16(s, e->src, e->src16, (size_t)e->src_layer * V));
         HIPCHK(ctx, pmx_launch_quantize_f16(s, e->ref, nullptr, (size_t)e->ref_pitch * H));
+    } else {
+        /* FP32 storage: each texel with its right neighbour's difference,
+         * so a bilinear row blend is one FMA */
+        HIPCHK(ctx, pmx_launch_pair_f32(s, e->src, e->src2, (size_t)e->src_layer * V));
     }
     /* reference box sums and NCC statistics (FP64) */
     double* dtmp = (double*)e->cost_cm; /* >= 2*W*H doubles when V >= 4; else allocate */
@@ -1013,6 +1029,14 @@ int pmx_engine_set_storage(pmx_engine* e, int storage) {
     if (storage == PMX_STORE_F16 && !e->src16)
         PMXCHK(dev_alloc(e->ctx, (void**)&e->src16,
                          sizeof(uint16_t) * (size_t)e->src_layer * e->V));
+    if (storage == PMX_STORE_F32 && !e->src2)
+        PMXCHK(dev_alloc(e->ctx, (void**)&e->src2, sizeof(float2) * (size_t)e->src_layer * e->V));
+    if (storage == PMX_STORE_F16 && e->src2) {
+        /* the FP32 pairs are rebuilt if FP32 storage is chosen again */
+        HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
+        dev_free(e->src2);
+        e->src2 = nullptr;
+    }
     if (storage != e->storage) e->prepared = false;
     e->storage = storage;
     return PMX_OK;

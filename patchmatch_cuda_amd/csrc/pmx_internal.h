@@ -42,10 +42,10 @@ struct pmx_ctx {
 struct PmxSweepArgs {
     const float* src;      /* view 0, texel (0,0) of the padded source block */
     const void* src_block; /* start of the padded source block */
-    long src_bytes;        /* its size (< 2^31: byte offsets are 32-bit) */
+    long src_bytes;        /* its size (< 2^32: byte offsets are 32-bit) */
     long src_layer;        /* texels between layers */
     int src_pitch;         /* texels between rows */
-    int src_h16;           /* 1: the block holds FP16 texels, 0: FP32 */
+    int src_h16;           /* 1: the block holds FP16 texels, 0: FP32 texel pairs */
     int srcW, srcH;        /* packed source size */
     const float* ref;      /* pre-blurred reference, W x H */
     int ref_pitch;         /* floats */
@@ -101,6 +101,8 @@ struct pmx_engine {
     float* src_raw = nullptr;   /* padded source block as uploaded */
     float* src = nullptr;       /* padded source block, pre-blurred */
     uint16_t* src16 = nullptr;  /* the same block as FP16 (storage PMX_STORE_F16) */
+    float2* src2 = nullptr;     /* the same block as FP32 texel pairs (t[x], t[x+1] - t[x])
+                                 * (storage PMX_STORE_F32) */
     int storage = PMX_STORE_F32;
     long src_layer = 0;
     int src_pitch = 0;          /* floats */
@@ -186,6 +188,7 @@ hipError_t pmx_launch_cost_volume(hipStream_t s, const PmxSweepArgs& a, int view
 hipError_t pmx_launch_sweep_batch(hipStream_t s, const PmxBatchLaunch& b, int* per_cu);
 hipError_t pmx_launch_zero_u32(hipStream_t s, unsigned* p, int n);
 hipError_t pmx_launch_quantize_f16(hipStream_t s, float* data, uint16_t* out, size_t n);
+hipError_t pmx_launch_pair_f32(hipStream_t s, const float* data, float2* out, size_t n);
 hipError_t pmx_launch_seed_depth(hipStream_t s, const float* prop, int prop_ld, float* depth,
                                  int depth_ld, int width, int height);
 hipError_t pmx_launch_gather_costs(hipStream_t s, const float* cost_rm, int cost_ld, float* out,

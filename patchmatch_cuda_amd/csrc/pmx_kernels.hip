@@ -454,13 +454,14 @@ struct PixelCtx {
 
 /* The padded source block as one raw buffer resource: 32-bit byte offsets,
  * descriptor in SGPRs.  H16: texels are FP16 (a bilinear row pair is one
- * 4-byte load), else FP32 (one 8-byte load).  Tex is what a row load
- * returns; it stays raw until the blend, so the conversion does not wait
- * on the load inside the fetch batch. */
+ * 4-byte load), else FP32 texel pairs: element x of a row holds t[x] and
+ * t[x+1] - t[x] (one 8-byte load, and the row blend is one FMA).  Tex is
+ * what a row load returns; it stays raw until the blend, so the conversion
+ * does not wait on the load inside the fetch batch. */
 template <bool H16>
 struct SrcBuf {
     using Tex = typename std::conditional<H16, uint32_t, float2>::type;
-    static constexpr uint32_t kShift = H16 ? 1u : 2u; /* log2 bytes per texel */
+    static constexpr uint32_t kShift = H16 ? 1u : 3u; /* log2 bytes per element */
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t layer_bytes, pitch_bytes;
     int srcW, srcH;
@@ -469,8 +470,9 @@ struct SrcBuf {
 template <bool H16>
 __device__ __forceinline__ SrcBuf<H16> src_buf(const PmxSweepArgs& a) {
     SrcBuf<H16> b;
-    b.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.src_block, (short)0, (int)a.src_bytes,
-                                               0x00020000);
+    /* the record count is a 32-bit field (blocks up to 4 GiB - 1) */
+    const uint32_t nrec = (uint32_t)min(a.src_bytes, 0xffffffffL);
+    b.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.src_block, (short)0, (int)nrec, 0x00020000);
     b.layer_bytes = (uint32_t)a.src_layer << SrcBuf<H16>::kShift;
     b.pitch_bytes = (uint32_t)a.src_pitch << SrcBuf<H16>::kShift;
     b.srcW = a.srcW;
@@ -518,7 +520,8 @@ __device__ __forceinline__ TapW tap_fetch(const SB& sb, const Hyp& q, float sx, 
     sx = __builtin_amdgcn_fmed3f(sx, q.lox, q.hix);
     sy = __builtin_amdgcn_fmed3f(sy, q.loy, q.hiy);
     const int ix = floor_i(sx), iy = floor_i(sy);
-    /* |iy| < 2^17 and the pitch < 2^17: a 24-bit multiply is exact */
+    /* |iy| < 2^23 and the pitch < 2^23 bytes (source images of at most
+     * 2^22 rows and 2^20 - 68 columns): signed 24-bit operands */
     /* iy * pitch + (ix << shift) + base as a shift-add and one signed
      * 24-bit multiply-add (the compiler otherwise re-associates it into
      * three instructions) */
@@ -536,11 +539,14 @@ __device__ __forceinline__ TapW tap_fetch(const SB& sb, const Hyp& q, float sx, 
     return TapW{__builtin_amdgcn_fractf(sx), __builtin_amdgcn_fractf(sy)};
 }
 
-/* Horizontal blend of a texel pair, (1 - al) lo + al hi, in FP32.  FP16
- * pairs (low half first) are blended as lo + al hi - al lo: two FMAs whose
- * FP16 operands the hardware widens itself (v_fma_mix_f32 selects either
- * half of the loaded word), so no conversion instructions are needed. */
-__device__ __forceinline__ float row_blend(float al, float2 t) { return fmaf(al, t.y - t.x, t.x); }
+/* Horizontal blend of a texel pair, (1 - al) lo + al hi, in FP32: an FP32
+ * pair element holds lo and hi - lo (the difference rounded to FP32 when the
+ * block is built, as it would be here), so the blend is lo + al (hi - lo) in
+ * one FMA.  FP16 pairs (low half first) are blended as lo + al hi - al lo:
+ * two FMAs whose FP16 operands the hardware widens itself (v_fma_mix_f32
+ * selects either half of the loaded word), so no conversion instructions
+ * are needed. */
+__device__ __forceinline__ float row_blend(float al, float2 t) { return fmaf(al, t.y, t.x); }
 __device__ __forceinline__ float row_blend(float al, uint32_t w) {
     const float lo = (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
     const float hi = (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
@@ -1194,6 +1200,16 @@ __global__ __launch_bounds__(64) void pmx_volume_kernel(PmxSweepArgs a, int view
 
 /* FP16 storage: rounds n floats to the nearest FP16 value (ties to even)
  * in place and, when out is not null, stores the FP16 bits there. */
+/* FP32 texel pairs (t[i], t[i+1] - t[i]) of a padded block of n texels;
+ * the last texel's neighbour is 0 (padding). */
+__global__ void pair_f32_kernel(const float* __restrict__ data, float2* __restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float t = data[i];
+    const float u = i + 1 < n ? data[i + 1] : 0.0f;
+    out[i] = make_float2(t, u - t);
+}
+
 __global__ void quantize_f16_kernel(float* __restrict__ data, uint16_t* __restrict__ out, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -2049,6 +2065,13 @@ hipError_t pmx_launch_cost_volume(hipStream_t s, const PmxSweepArgs& a, int view
     if (Wn <= 16) return launch_volume_t<16, 4, 0>(s, a, view0, view1, out, W0, H0);
     if (Wn <= 32) return launch_volume_t<32, 16, 0>(s, a, view0, view1, out, W0, H0);
     return launch_volume_t<64, 44, 0>(s, a, view0, view1, out, W0, H0);
+}
+
+hipError_t pmx_launch_pair_f32(hipStream_t s, const float* data, float2* out, size_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pair_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, data,
+                       out, n);
+    return hipGetLastError();
 }
 
 hipError_t pmx_launch_quantize_f16(hipStream_t s, float* data, uint16_t* out, size_t n) {

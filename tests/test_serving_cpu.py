@@ -33,13 +33,16 @@ def test_footprint_closed_form():
     fp = capi.engine_footprint(rp, W, H, [(W, H)] * V, capi.STORE_F16, proposal=True)
     pitch = (W + 4 + 63) // 64 * 64
     layer = pitch * (H + 4)
-    # the FP16 block is 2 bytes per padded texel per view
-    assert f16 - f32 == (2 * layer * V + 255) // 256 * 256
+    # FP16 storage holds 2 bytes per padded texel per view in place of the
+    # 8-byte FP32 texel pairs
+    g = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    assert f32 - f16 == g(8 * layer * V) - g(2 * layer * V)
     # proposal planes: depth and spread, row- and column-major
     ld_r, ld_c = (W + 63) // 64 * 64, (H + 63) // 64 * 64
     assert fp - f16 == 2 * ((4 * ld_r * H + 255) // 256 * 256) + 2 * ((4 * ld_c * W + 255) // 256 * 256)
-    # dominant terms: two FP32 source blocks, FP64 messages, two cost layouts
-    dominant = 2 * 4 * layer * V + 8 * W * H * V + 2 * 4 * W * H * V
+    # dominant terms: two FP32 source blocks and the texel pairs, the
+    # messages and the output costs, two cost layouts
+    dominant = 2 * 4 * layer * V + 8 * layer * V + 8 * W * H * V + 2 * 4 * W * H * V
     assert dominant < f32 < dominant * 1.5
     # more views, more bytes
     assert capi.engine_footprint(rp, W, H, [(W, H)] * (V + 1)) > f32
