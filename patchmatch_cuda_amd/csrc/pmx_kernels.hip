@@ -117,6 +117,24 @@ __device__ __forceinline__ float wave_sum(float v, int n) {
 }
 
 /* A wave-uniform value in a scalar register (its load is waited for here). */
+/* Column-uniform values that only vector code uses, held in vector
+ * registers: the scalar registers of the sweep are the scarce ones (the
+ * compiler would otherwise keep them there and spill others). */
+template <class T>
+__device__ __forceinline__ void to_vgpr(T& v) {
+    asm volatile("" : "+v"(v));
+}
+/* A global-memory pointer held in vector registers (the address space is
+ * kept, so the accesses stay global loads and stores). */
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> vglobal(T* p) {
+    gptr<T> g = (gptr<T>)p;
+    asm volatile("" : "+v"(g));
+    return g;
+}
+
 __device__ __forceinline__ float uniform_f(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
@@ -616,11 +634,11 @@ __device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const SB&
 
 /* A hypothesis' projection as four float4 in LDS (computed once, read by
  * every lane group that sums its window). */
-__device__ __forceinline__ void store_hyp(float4* h, const Hyp& q) {
+__device__ __forceinline__ void store_hyp(float4* h, const Hyp& q, int step = 1) {
     h[0] = make_float4(q.ux, q.uy, q.vx, q.vy);
-    h[1] = make_float4(q.cx, q.cy, q.fxc, q.fyc);
-    h[2] = make_float4(q.lox, q.hix, q.loy, q.hiy);
-    h[3] = make_float4(__uint_as_float(q.base), 0.0f, 0.0f, 0.0f);
+    h[step] = make_float4(q.cx, q.cy, q.fxc, q.fyc);
+    h[2 * step] = make_float4(q.lox, q.hix, q.loy, q.hiy);
+    h[3 * step] = make_float4(__uint_as_float(q.base), 0.0f, 0.0f, 0.0f);
 }
 
 __device__ __forceinline__ Hyp load_hyp(const float4* h) {
@@ -1361,9 +1379,17 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     const SrcBuf<H16> sb = src_buf<H16>(a);
     const MsgConst mc = msg_const(a.alpha2);
 
-    float* dcol = a.depth + (size_t)c * a.depth_ld;
-    float* ccol = a.cost + (size_t)c * a.cost_ld;
-    float* scol = a.spm + (size_t)c * a.spm_ld;
+    /* this column's depth, costs, backward messages, proposal and window
+     * statistics (x = c in a plain sweep, y = c in a rotated one) */
+    const gptr<float> dcol = vglobal(a.depth + (size_t)c * a.depth_ld);
+    const gptr<float> ccol = vglobal(a.cost + (size_t)c * a.cost_ld);
+    const gptr<float> scol = vglobal(a.spm + (size_t)c * a.spm_ld);
+    const gptr<const float> pcol = vglobal(prp ? a.prop + (size_t)c * a.prop_ld : a.depth);
+    const gptr<const float> qcol = vglobal(prp ? a.prop_spread + (size_t)c * a.prop_ld : a.depth);
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    const gptr<const f32x2> stcol =
+        vglobal((const f32x2*)(a.stats + (rot ? (size_t)c * a.stats_pitch : (size_t)c)));
+    const int ststep = rot ? 1 : a.stats_pitch;
 
     /* generator of this column: row 0 (plain) or column 0 (rotated) */
     pmx_rng_state* rp = rot ? (pmx_rng_state*)((char*)a.rng + (size_t)c * a.rng_pitch)
@@ -1406,7 +1432,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         const float e = sentinel_emission(cst, is_view, lane, rng);
         m = emission_lr(e, mc); /* G(0) = 0 */
     }
-    float d_former = dcol[r0];
+    float d_former = uniform_f(dcol[r0]);
 
     const int qx_col = rot ? 0 : max(0, a.hw - c);
     const int qy_col = rot ? max(0, a.hw - c) : 0;
@@ -1428,10 +1454,13 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         pf_beta = scol[(size_t)rr * V + vl];
         pf_depth = dcol[rr];
         if (prp) {
-            pf_prop = a.prop[(size_t)c * a.prop_ld + rr];
-            pf_spread = a.prop_spread[(size_t)c * a.prop_ld + rr];
+            pf_prop = pcol[rr];
+            pf_spread = qcol[rr];
         }
-        pf_stat = a.stats[(size_t)y * a.stats_pitch + x];
+        {
+            const f32x2 st2 = stcol[(size_t)rr * ststep];
+            pf_stat = make_float2(st2.x, st2.y);
+        }
         if constexpr (PF <= 8)
             load_col_window<PF>(cwin, rr, pf_win);
         else
@@ -1443,18 +1472,18 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     if (Hs_rows > 1) D = store_window<PF>(winbuf, pf_win, pf_stat.x, Wn, lane, need_d);
 
     unsigned long long n_evals = 0, n_samples = 0;
+    to_vgpr(n_evals);
+    to_vgpr(n_samples);
     int pb = 0; /* pass counter: parity selects the partial-sum buffer */
 
-    /* one phase of hypotheses tview/tdep[0..nt) -> tout[0..nt) */
+    /* one phase of hypotheses tview/tdep[0..nt) -> tout[0..nt); in the
+     * small-view kernels the hypotheses' projections hypb[0..nt) */
     auto run_phase = [&](int nt, const float* win, float* tout, const PixelCtx& pc) {
         if constexpr (SM) {
-            /* at most 3 x 7 hypotheses: each projection is computed once, by
-             * one lane, the passes leave their window sums in `part` (in the
-             * order a per-pass finish would combine them), and one lane per
-             * hypothesis forms its 1 - NCC */
-            for (int t = (int)threadIdx.x; t < nt; t += 64 * NW)
-                store_hyp(hypb + 4 * t, make_hyp(Hs, sb, tview[t], tdep[t], pc));
-            lds_barrier();
+            /* at most 3 x 7 hypotheses, each projection computed once, by one
+             * lane, before the phase: the passes leave their window sums in
+             * `part` (in the order a per-pass finish would combine them), and
+             * one lane per hypothesis forms its 1 - NCC */
             for (int t0 = 0; t0 < nt; t0 += NGT)
                 eval_pass<G, WN, NW, NB, true, true>(a, Hs, sb, win, wave, lane, min(NGT, nt - t0), nullptr,
                                                      nullptr, nullptr, part + t0 * NW, pc, hypb + 4 * t0);
@@ -1592,7 +1621,9 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             const uint64_t last = 1ull << (V - 1);
             auto draw = [&](int t) {
                 const uint32_t kt = (xw_next(rng) >> 8) + 1u;
-                const uint64_t hit = __ballot(kt <= cvi) | last;
+                /* the compare's lane mask as is (a ballot of the bool
+                 * would select and compare it again) */
+                const uint64_t hit = __builtin_amdgcn_uicmp(kt, cvi, 37 /* ICMP_ULE */) | last;
                 const int id = (int)__builtin_ctzll(hit);
                 sel |= 1ull << id;
                 cnt += (lane == id) ? 1 : 0;
@@ -1643,21 +1674,42 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         /* a learned proposal is a fourth candidate on the same views */
         const bool has_prop = prp && d_prop > 0.0f;
         const int per = (same ? 1 : 2) + (has_prop ? 1 : 0);
-        if (in_sel) {
-            int o = per * srank;
-            if (!same) {
+        /* Small-view kernels: lane 16 k + v projects view v at the former
+         * (k = 0), random (1) or proposal (2) depth, every view and depth a
+         * phase of this row can ask for; phase B's hypotheses are stored in
+         * task order now, phase C's from the same registers once the winner
+         * is known. */
+        Hyp hq;
+        const int hk = lane >> 4, hv = lane & 15;
+        if constexpr (SM) {
+            const float dk = hk == 0 ? d_former : (hk == 1 || !has_prop ? d_rand : d_prop);
+            hq = make_hyp(Hs, sb, hv < V ? hv : 0, dk, pc);
+            const bool on = lead && hv < V && ((sel >> hv) & 1ull) &&
+                            (hk == 1 || (hk == 0 && !same) || (hk == 2 && has_prop));
+            const int rk = __popc((uint32_t)sel & ((1u << hv) - 1u));
+            const int pos = hk == 0 ? 0 : (same ? 0 : 1) + (hk == 2 ? 1 : 0);
+            /* lanes without a hypothesis write four times to a float4 of
+             * their own in the task table, which these kernels do not use
+             * (no branch) */
+            store_hyp(on ? hypb + 4 * (per * rk + pos) : (float4*)tview + lane, hq, on ? 1 : 0);
+            lds_barrier();
+        } else {
+            if (in_sel) {
+                int o = per * srank;
+                if (!same) {
+                    tview[o] = lane;
+                    tdep[o++] = d_former;
+                }
                 tview[o] = lane;
-                tdep[o++] = d_former;
+                tdep[o++] = d_rand;
+                if (has_prop) {
+                    tview[o] = lane;
+                    tdep[o] = d_prop;
+                }
             }
-            tview[o] = lane;
-            tdep[o++] = d_rand;
-            if (has_prop) {
-                tview[o] = lane;
-                tdep[o] = d_prop;
-            }
+            if (NW == 1) lds_barrier();
         }
         const int nt = per * U;
-        if (NW == 1) lds_barrier();
 
         /* phase B: former and random depth (and the proposal) on the sampled views */
         run_phase(nt, win, tres, pc);
@@ -1705,11 +1757,19 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             const int nrest = V - U;
             const bool in_rest = is_view && !in_sel;
             const int rrank = __popcll(rest & ((1ull << lane) - 1ull));
-            if (in_rest) {
-                tview[rrank] = lane;
-                tdep[rrank] = best;
+            if constexpr (SM) {
+                const int kc = idx == 0 ? 0 : (idx == 2 ? 1 : 2);
+                const bool on = lead && hk == kc && hv < V && !((sel >> hv) & 1ull);
+                const int rk = __popc((uint32_t)rest & ((1u << hv) - 1u));
+                store_hyp(on ? hypb + 4 * rk : (float4*)tview + lane, hq, on ? 1 : 0);
+                lds_barrier();
+            } else {
+                if (in_rest) {
+                    tview[rrank] = lane;
+                    tdep[rrank] = best;
+                }
+                if (NW == 1) lds_barrier();
             }
-            if (NW == 1) lds_barrier();
             run_phase(nrest, win, tres + kTasks, pc);
             if (in_sel) cnew = (idx == 0) ? nf : (idx == 2 ? nr : np);
             if (in_rest) cnew = tres[kTasks + rrank];
