@@ -820,6 +820,21 @@ __device__ __forceinline__ float ncc_finish(float A, float B, float C, float K,
     return (float)(1.0 - ((double)C + (double)K * (double)pc.D) * r);
 }
 
+/* ncc_finish without branches (every lane computes the normalisation; the
+ * special cases are selected at the end, with the same values). */
+__device__ __forceinline__ float ncc_finish_sel(float A, float B, float C, float K,
+                                                const PixelCtx& pc) {
+    double c2 = (double)B - (double)A * (double)A * pc.inv_n;
+    c2 = c2 < 0.00001 ? 0.0 : c2;
+    const double p = (double)pc.c1 * c2;
+    double r = __builtin_amdgcn_rsq(p);
+    const double hp = 0.5 * p;
+    r = r * fma(-hp * r, r, 1.5);
+    r = r * fma(-hp * r, r, 1.5);
+    const float v = (float)(1.0 - ((double)C + (double)K * (double)pc.D) * r);
+    return (K == 0.0f && B == 0.0f) ? kSentinel : (p == 0.0 ? 1.0f : v);
+}
+
 /*
  * One pass over nt hypotheses (tview[t], tdep[t]) of one pixel on the NW
  * wavefronts of a workgroup (NW*64/G lane groups).  One group per
@@ -916,13 +931,14 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
             C += __shfl_xor(C, m, kLanes);
         }
     }
-    if (k <= NG) {
-        if (active && sub == last && gl == 0) {
-            if (PRE)
-                part[t * NW] = make_float4(A, B, C, K);
-            else
-                tout[t] = ncc_finish(A, B, C, K, pc);
-        }
+    if constexpr (PRE) {
+        /* every lane stores: the group's writer its sums, the others to a
+         * spare slot of their own, part[40 NW + lane], past the at most
+         * 24 NW slots of a phase's hypotheses (no branch) */
+        const bool wr = active && (k <= NG ? sub == last : (sub & (NG - 1)) == last) && gl == 0;
+        part[wr ? t * NW + (k <= NG ? 0 : sub / NG) : 40 * NW + lane] = make_float4(A, B, C, K);
+    } else if (k <= NG) {
+        if (active && sub == last && gl == 0) tout[t] = ncc_finish(A, B, C, K, pc);
     } else if (active && (sub & (NG - 1)) == last && gl == 0) {
         part[t * NW + sub / NG] = make_float4(A, B, C, K);
     }
@@ -1488,19 +1504,26 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                 eval_pass<G, WN, NW, NB, true, true>(a, Hs, sb, win, wave, lane, min(NGT, nt - t0), nullptr,
                                                      nullptr, nullptr, part + t0 * NW, pc, hypb + 4 * t0);
             lds_barrier();
-            for (int t = (int)threadIdx.x; t < nt; t += 64 * NW) {
-                const int ntp = min(NGT, nt - (t & ~(NGT - 1)));
-                const int lk = __builtin_ctz((unsigned)NGT) -
-                               (ntp > 1 ? 32 - __builtin_clz((unsigned)(ntp - 1)) : 0);
-                const int slices = (1 << lk) > NG ? (1 << lk) / NG : 1;
+            /* lane t of wave 0 finishes hypothesis t (t < nt <= 24); the
+             * lanes past nt repeat the last one into entries nobody reads
+             * (tout has 64), so there is no branch */
+            if (wave == 0 && nt > 0) {
+                const int t = min(lane, nt - 1);
                 float4 s = part[t * NW];
-                for (int w = 1; w < slices; w++) {
-                    const float4 q = part[t * NW + w];
-                    s.x += q.x;
-                    s.y += q.y;
-                    s.z += q.z;
+                if constexpr (NW > 1) {
+                    /* hypotheses split over the waves: partials in slice order */
+                    const int ntp = min(NGT, nt - (t & ~(NGT - 1)));
+                    const int lk = __builtin_ctz((unsigned)NGT) -
+                                   (ntp > 1 ? 32 - __builtin_clz((unsigned)(ntp - 1)) : 0);
+                    const int slices = (1 << lk) > NG ? (1 << lk) / NG : 1;
+                    for (int w = 1; w < slices; w++) {
+                        const float4 q = part[t * NW + w];
+                        s.x += q.x;
+                        s.y += q.y;
+                        s.z += q.z;
+                    }
                 }
-                tout[t] = ncc_finish(s.x, s.y, s.z, s.w, pc);
+                tout[lane] = ncc_finish_sel(s.x, s.y, s.z, s.w, pc);
             }
             lds_barrier();
             return;
@@ -1714,7 +1737,15 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         /* phase B: former and random depth (and the proposal) on the sampled views */
         run_phase(nt, win, tres, pc);
         float nf = 0.0f, nr = 0.0f, np = 0.0f;
-        if (in_sel) {
+        if constexpr (SM) {
+            /* every lane reads (per srank + 2 < 64, inside tres), the
+             * selected views keep theirs */
+            const int o = per * srank;
+            const float t0 = tres[o], t1 = tres[o + 1], t2 = tres[o + 2];
+            nf = in_sel ? (same ? cst : t0) : 0.0f;
+            nr = in_sel ? (same ? t0 : t1) : 0.0f;
+            np = in_sel && has_prop ? (same ? t1 : t2) : 0.0f;
+        } else if (in_sel) {
             int o = per * srank;
             nf = same ? cst : tres[o++];
             nr = tres[o++];
@@ -1771,8 +1802,13 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                 if (NW == 1) lds_barrier();
             }
             run_phase(nrest, win, tres + kTasks, pc);
-            if (in_sel) cnew = (idx == 0) ? nf : (idx == 2 ? nr : np);
-            if (in_rest) cnew = tres[kTasks + rrank];
+            if constexpr (SM) {
+                const float tr = tres[kTasks + rrank]; /* rrank < 16 */
+                cnew = in_sel ? ((idx == 0) ? nf : (idx == 2 ? nr : np)) : (in_rest ? tr : cnew);
+            } else {
+                if (in_sel) cnew = (idx == 0) ? nf : (idx == 2 ? nr : np);
+                if (in_rest) cnew = tres[kTasks + rrank];
+            }
             if (lead && is_view) ccol[(size_t)r * V + lane] = cnew;
             n_evals += (unsigned long long)nrest;
         }
