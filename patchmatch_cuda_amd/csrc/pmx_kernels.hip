@@ -843,14 +843,16 @@ __device__ __forceinline__ float ncc_finish_sel(float A, float B, float C, float
  * If k <= groups per wave, the hypothesis stays inside one wave and its
  * 1 - NCC is written to tout[t]; otherwise each wave writes its partial
  * sums to part[t * NW + slice] and the caller combines them after a
- * barrier (combine_pass).  Returns k.
+ * barrier (combine_pass).  PRE: the caller passes each lane group's
+ * projection in qpre and every group's sums go to part (see below).
+ * Returns k.
  */
 template <int G, int WN, int NW, int NB = 0, bool INT = false, bool PRE = false, class SB>
 __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __restrict__ Hm,
                                          const SB& sb, const float* __restrict__ win,
                                          int wave, int lane, int nt, const int* tview,
                                          const float* tdep, float* tout, float4* part,
-                                         const PixelCtx& pc, const float4* hyps = nullptr) {
+                                         const PixelCtx& pc, const Hyp* qpre = nullptr) {
     constexpr int NG = kLanes / G;
     constexpr int NGT = NG * NW;
     /* tap rows in flight per lane: a whole column with one wave per
@@ -867,7 +869,7 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
     const bool active = t < nt;
     float A = 0.0f, B = 0.0f, C = 0.0f, K = 0.0f;
     if (active) {
-        const Hyp q = PRE ? load_hyp(hyps + 4 * t) : make_hyp(Hm, sb, tview[t], tdep[t], pc);
+        const Hyp q = PRE ? *qpre : make_hyp(Hm, sb, tview[t], tdep[t], pc);
         const int R = (Wn + k - 1) >> lk;
         const int r0 = max(sub * R, pc.dy_lo);
         const int r1 = min(min((sub + 1) * R, Wn), pc.dy_hi + 1);
@@ -1036,7 +1038,8 @@ __device__ __forceinline__ ColWindow<PF> col_window(const PmxSweepArgs& a, bool 
         const int dxi = idx - dyi * Wn;
         const int dv = (rot ? dxi : dyi) - a.hw;
         const int bf = c + (rot ? dyi : dxi) - a.hw; /* fixed coordinate */
-        const bool inF = (unsigned)bf < (unsigned)Lf;
+        /* positions past the window hold 0 (the zero rows after it) */
+        const bool inF = (unsigned)bf < (unsigned)Lf && idx < Wn * Wn;
         const bool rdF = inF && bf - qf >= 0 && idx < Wn * Wn;
         cw.dv[k] = dv;
         cw.off[k] = 4 * (rot ? (bf - qf) * a.ref_pitch + dv : dv * a.ref_pitch + (bf - qf));
@@ -1069,15 +1072,17 @@ __device__ __forceinline__ void load_col_window(const ColWindow<PF>& cw, int r, 
 /* Writes a loaded window as R' = Rq - mean into LDS (0 outside the image).
  * With need_d (the tile-index shift applies) returns the wave sum D of R'
  * over the in-bounds positions; otherwise D is exactly 0 (the in-bounds
- * values sum to the box sum) and 0 is returned. */
-template <int PF>
+ * values sum to the box sum) and 0 is returned.  ALL: every position is
+ * written, those past the window with 0 (their flags are clear), which the
+ * caller's buffer must hold (64 PF <= 2 Wn^2, window and zero rows). */
+template <int PF, bool ALL = false>
 __device__ __forceinline__ float store_window(float* __restrict__ win, const WinVals<PF>& w,
                                               float mu, int Wn, int lane, bool need_d) {
     float d = 0.0f;
 #pragma unroll
     for (int k = 0; k < PF; k++) {
         const int idx = lane + kLanes * k;
-        if (idx < Wn * Wn) {
+        if (ALL || idx < Wn * Wn) {
             const float v = ((w.rd >> k) & 1u) ? w.v[k] : 0.0f;
             const float r = ((w.in >> k) & 1u) ? v - mu : 0.0f;
             win[idx] = r;
@@ -1485,7 +1490,9 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     if (Hs_rows > 1) prefetch(r);
     int buf = 0;
     float D = 0.0f;
-    if (Hs_rows > 1) D = store_window<PF>(winbuf, pf_win, pf_stat.x, Wn, lane, need_d);
+    /* column windows of a fixed size fill their buffer without a branch */
+    constexpr bool kWinAll = PF <= 8 && WN > 0 && kLanes * PF <= 2 * WN * WN;
+    if (Hs_rows > 1) D = store_window<PF, kWinAll>(winbuf, pf_win, pf_stat.x, Wn, lane, need_d);
 
     unsigned long long n_evals = 0, n_samples = 0;
     to_vgpr(n_evals);
@@ -1500,9 +1507,14 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
              * lane, before the phase: the passes leave their window sums in
              * `part` (in the order a per-pass finish would combine them), and
              * one lane per hypothesis forms its 1 - NCC */
-            for (int t0 = 0; t0 < nt; t0 += NGT)
-                eval_pass<G, WN, NW, NB, true, true>(a, Hs, sb, win, wave, lane, min(NGT, nt - t0), nullptr,
-                                                     nullptr, nullptr, part + t0 * NW, pc, hypb + 4 * t0);
+            for (int t0 = 0; t0 < nt; t0 += NGT) {
+                const int ntp = min(NGT, nt - t0);
+                const int lk = __builtin_ctz((unsigned)NGT) -
+                               (ntp > 1 ? 32 - __builtin_clz((unsigned)(ntp - 1)) : 0);
+                const Hyp q = load_hyp(hypb + 4 * (t0 + min((wave * NG + lane / G) >> lk, ntp - 1)));
+                eval_pass<G, WN, NW, NB, true, true>(a, Hs, sb, win, wave, lane, ntp, nullptr,
+                                                     nullptr, nullptr, part + t0 * NW, pc, &q);
+            }
             lds_barrier();
             /* lane t of wave 0 finishes hypothesis t (t < nt <= 24); the
              * lanes past nt repeat the last one into entries nobody reads
@@ -1653,8 +1665,19 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                 ids6 |= (uint64_t)id << (6 * (t - 1));
             };
             if constexpr (SM) {
-                /* a loop: no per-draw predicates held across the row loop */
-                for (int t = 1; t <= a.S; t++) draw(t);
+                /* a loop: no per-draw predicates held across the row loop;
+                 * the ids are packed only when they are recorded */
+                if (a.ids) {
+                    for (int t = 1; t <= a.S; t++) draw(t);
+                } else {
+                    for (int t = 1; t <= a.S; t++) {
+                        const uint32_t kt = (xw_next(rng) >> 8) + 1u;
+                        const uint64_t hit = __builtin_amdgcn_uicmp(kt, cvi, 37 /* ICMP_ULE */) | last;
+                        const int id = (int)__builtin_ctzll(hit);
+                        sel |= 1ull << id;
+                        cnt += (lane == id) ? 1 : 0;
+                    }
+                }
             } else {
 #pragma unroll
                 for (int t = 1; t < 8; t++)
@@ -1828,7 +1851,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         /* next window into the other buffer */
         if (k + 1 < Hs_rows) {
             buf ^= 1;
-            D = store_window<PF>(winbuf + buf * WW, pf_win, pf_stat.x, Wn, lane, need_d);
+            D = store_window<PF, kWinAll>(winbuf + buf * WW, pf_win, pf_stat.x, Wn, lane, need_d);
         }
     }
     if (lead && lane == 0) {
