@@ -82,6 +82,12 @@ typedef struct pmxo_trace {
     double*  cdf;          /* Ws*Hs*V: the normalised view CDF the row's draws read, [r][c][v] */
     double*  rand_depth;   /* Ws*Hs: the row's random depth hypothesis */
     double*  cand_cost;    /* Ws*Hs*4: averaged costs of former, current, random, proposal */
+    /* Optional: the largest posterior numerator m_f * beta over the row's
+     * views (+inf where not processed).  Where it is below ~1e-300 the view
+     * CDF is decided by FP64 underflow (all numerators 0: a NaN CDF) or
+     * read from denormal numerators, a regime no other precision follows
+     * decision for decision. */
+    double*  post_max;
 } pmxo_trace;
 
 typedef struct pmxo_problem pmxo_problem;

@@ -41,8 +41,19 @@ VTOL = 2e-5
 CTOL = 2e-5
 
 
+# Below this largest posterior numerator the oracle's view CDF comes from
+# FP64 underflow (a NaN CDF when every numerator is 0) or from denormal
+# numerators; the engine decides that regime in the log domain, so the two
+# need not draw the same view there (it arises only at the left/top border
+# columns, whose tile-index quirk gives costs far outside [0, 2]).
+POST_TINY = 1e-300
+
+
 def fragile_mask(trace, vtol=VTOL, ctol=CTOL):
-    return (trace["view_margin"] < vtol) | (trace["cost_margin"] < ctol) | (trace["near_clamp"] > 0)
+    m = (trace["view_margin"] < vtol) | (trace["cost_margin"] < ctol) | (trace["near_clamp"] > 0)
+    if "post_max" in trace:
+        m |= trace["post_max"] < POST_TINY
+    return m
 
 
 def compare_sweep(gpu_depth_s, ora_depth_s, gpu_ids_s, ora_ids_s, trace, down,
