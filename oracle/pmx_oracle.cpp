@@ -567,7 +567,7 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
             if (trace->rand_depth) trace->rand_depth[i] = NAN;
             for (int v = 0; trace->cdf && v < V; v++) trace->cdf[i * V + v] = NAN;
             for (int q = 0; trace->cand_cost && q < 4; q++) trace->cand_cost[i * 4 + q] = NAN;
-            if (trace->post_max) trace->post_max[i] = INFINITY;
+            if (trace->lpost_max) trace->lpost_max[i] = INFINITY;
         }
     }
 
@@ -610,11 +610,11 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
             load_costs(r);
             message_forward(m.data(), costs.data(), V, alpha2, rng);
             /* posterior and its CDF (src/patchMatch.cu:909-923). */
-            double z0max = 0.0;
+            double lz0max = -INFINITY;
             for (int v = 0; v < V; v++) {
                 const double b = spm[(size_t)v * Hs + r];
                 const double z0 = m[v] * b;
-                z0max = std::fmax(z0max, z0);
+                lz0max = std::fmax(lz0max, std::log(m[v]) + std::log(b));
                 const double z1 = (1.0 - m[v]) * (1.0 - b);
                 cdf[v] = z0 / (z0 + z1);
             }
@@ -685,7 +685,7 @@ void pmxo_sweep(const pmxo_problem* pb, double* depth_s, double* cost_s,
                 if (trace->choice) trace->choice[pi] = (uint8_t)idx;
                 if (trace->near_clamp) trace->near_clamp[pi] = clamp_flag ? 1 : 0;
                 if (trace->rand_depth) trace->rand_depth[pi] = d_rand;
-                if (trace->post_max) trace->post_max[pi] = z0max;
+                if (trace->lpost_max) trace->lpost_max[pi] = lz0max;
                 for (int v = 0; trace->cdf && v < V; v++) trace->cdf[pi * V + v] = cdf[v];
                 for (int q = 0; trace->cand_cost && q < 4; q++)
                     trace->cand_cost[pi * 4 + q] = q < ncand ? cost3[q] : NAN;

@@ -82,12 +82,13 @@ typedef struct pmxo_trace {
     double*  cdf;          /* Ws*Hs*V: the normalised view CDF the row's draws read, [r][c][v] */
     double*  rand_depth;   /* Ws*Hs: the row's random depth hypothesis */
     double*  cand_cost;    /* Ws*Hs*4: averaged costs of former, current, random, proposal */
-    /* Optional: the largest posterior numerator m_f * beta over the row's
-     * views (+inf where not processed).  Where it is below ~1e-300 the view
-     * CDF is decided by FP64 underflow (all numerators 0: a NaN CDF) or
-     * read from denormal numerators, a regime no other precision follows
-     * decision for decision. */
-    double*  post_max;
+    /* Optional: log of the largest posterior numerator m_f * beta over the
+     * row's views, as log m_f + log beta (+inf where not processed, -inf
+     * where a message is exactly 0).  Between about -745 and -708 the FP64
+     * numerators are denormal or round to 0 (every one 0: a NaN CDF), a
+     * regime in which no other precision follows the view draws decision
+     * for decision. */
+    double*  lpost_max;
 } pmxo_trace;
 
 typedef struct pmxo_problem pmxo_problem;

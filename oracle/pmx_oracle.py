@@ -37,7 +37,7 @@ class _Trace(ctypes.Structure):
     _fields_ = [("view_margin", ctypes.c_void_p), ("cost_margin", ctypes.c_void_p),
                 ("choice", ctypes.c_void_p), ("near_clamp", ctypes.c_void_p),
                 ("cdf", ctypes.c_void_p), ("rand_depth", ctypes.c_void_p),
-                ("cand_cost", ctypes.c_void_p), ("post_max", ctypes.c_void_p)]
+                ("cand_cost", ctypes.c_void_p), ("lpost_max", ctypes.c_void_p)]
 
 
 _lib = None
@@ -266,13 +266,13 @@ class Problem:
         if trace:
             tr = dict(view_margin=np.zeros((Hs, Ws)), cost_margin=np.zeros((Hs, Ws)),
                       choice=np.zeros((Hs, Ws), np.uint8), near_clamp=np.zeros((Hs, Ws), np.uint8),
-                      post_max=np.zeros((Hs, Ws)))
+                      lpost_max=np.zeros((Hs, Ws)))
             if trace == "full":
                 tr.update(cdf=np.zeros((Hs, Ws, self.V)), rand_depth=np.zeros((Hs, Ws)),
                           cand_cost=np.zeros((Hs, Ws, 4)))
             trs = _Trace(_p(tr["view_margin"]), _p(tr["cost_margin"]), _p(tr["choice"]),
                          _p(tr["near_clamp"]), _p(tr.get("cdf")), _p(tr.get("rand_depth")),
-                         _p(tr.get("cand_cost")), _p(tr["post_max"]))
+                         _p(tr.get("cand_cost")), _p(tr["lpost_max"]))
         lib().pmxo_sweep(self._h, _p(depth_s), _p(cost_s), _p(states), int(down), int(rotated), S,
                          _p(ids), ctypes.byref(trs) if trs is not None else None)
         return ids, tr

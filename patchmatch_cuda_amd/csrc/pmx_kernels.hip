@@ -1295,12 +1295,20 @@ __device__ __forceinline__ MsgConst msg_const(double alpha2) {
     return mc;
 }
 
+/* Lanes whose cost is the sentinel, among the view lanes vmask (the
+ * compare's own lane mask: a ballot of the bool would select and compare it
+ * again). */
+__device__ __forceinline__ uint64_t sentinel_lanes(float cost, uint64_t vmask) {
+    return __builtin_amdgcn_fcmpf(cost, kSentinel, 1 /* FCMP_OEQ */) & vmask;
+}
+
 /* Sentinel draws: every view whose stored cost is exactly 2.0 consumes one
  * uniform, in view order, and uses 2u as its cost (src/patchMatch.cu:680-682,
- * :706-707). */
-__device__ __forceinline__ float sentinel_emission(float cost, bool is_view, int lane, Xw& rng) {
+ * :706-707).  vmask: the lanes of the views. */
+__device__ __forceinline__ float sentinel_emission(float cost, bool is_view, int lane, Xw& rng,
+                                                   uint64_t vmask) {
     const bool sent = is_view && (cost == kSentinel);
-    const uint64_t mask = __ballot(sent);
+    const uint64_t mask = sentinel_lanes(cost, vmask);
     const int k = __popcll(mask);
     float e = cost;
     if (k) {
@@ -1321,7 +1329,10 @@ __device__ __forceinline__ float emission_lr(float c, const MsgConst& mc) {
 /* G(l) = sign(l) log((fix + chg t) / (chg + fix t)), t = e^-|l| (no overflow) */
 __device__ __forceinline__ float trans_g(float l) {
     const float t = __expf(-fabsf(l));
-    const float g = __logf(fmaf(kChange, t, kFix)) - __logf(fmaf(kFix, t, kChange));
+    /* both arguments lie in [min(fix, chg), fix + chg]: the hardware log2
+     * (no denormal handling needed), one scaling of the difference */
+    const float g = (__builtin_amdgcn_logf(fmaf(kChange, t, kFix)) -
+                     __builtin_amdgcn_logf(fmaf(kFix, t, kChange))) * 0.693147180559945309f;
     return copysignf(g, l);
 }
 
@@ -1396,6 +1407,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     const int V = a.V, Hs_rows = a.Hs;
     const bool is_view = lane < V;
     const int vl = is_view ? lane : 0;
+    const uint64_t vmask_v = (V >= 64) ? ~0ull : ((1ull << V) - 1ull); /* the view lanes */
     const float inv_wn = 1.0f / (float)Wn;
     const SrcBuf<H16> sb = src_buf<H16>(a);
     const MsgConst mc = msg_const(a.alpha2);
@@ -1428,7 +1440,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
             const float cst = cn;
             if (k + 1 < Hs_rows) cn = ccol[(size_t)(rb + sbs) * V + vl];
             if (lead && is_view) scol[(size_t)rb * V + lane] = m;
-            const float e = sentinel_emission(cst, is_view, lane, rng);
+            const float e = sentinel_emission(cst, is_view, lane, rng, vmask_v);
             m = trans_g(m + emission_lr(e, mc));
         }
     }
@@ -1450,7 +1462,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     /* ---- forward message at the first row ---- */
     {
         const float cst = ccol[(size_t)r0 * V + vl];
-        const float e = sentinel_emission(cst, is_view, lane, rng);
+        const float e = sentinel_emission(cst, is_view, lane, rng, vmask_v);
         m = emission_lr(e, mc); /* G(0) = 0 */
     }
     float d_former = uniform_f(dcol[r0]);
@@ -1585,7 +1597,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
         /* forward message with the stored costs, posterior, CDF
          * (src/patchMatch.cu:901-923) */
         const float g_prev = trans_g(m); /* shared with the refresh below */
-        const float mf = g_prev + emission_lr(sentinel_emission(cst, is_view, lane, rng), mc);
+        const float mf = g_prev + emission_lr(sentinel_emission(cst, is_view, lane, rng, vmask_v), mc);
         /* posterior log-odds of each view; the normalised CDF of its
          * probabilities (inclusive prefix sum over lanes 0..V-1 divided by
          * the total).  Where every posterior is below FP64's smallest
@@ -1670,13 +1682,24 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                 if (a.ids) {
                     for (int t = 1; t <= a.S; t++) draw(t);
                 } else {
-                    for (int t = 1; t <= a.S; t++) {
+                    auto draw1 = [&]() {
                         const uint32_t kt = (xw_next(rng) >> 8) + 1u;
                         const uint64_t hit = __builtin_amdgcn_uicmp(kt, cvi, 37 /* ICMP_ULE */) | last;
                         const int id = (int)__builtin_ctzll(hit);
                         sel |= 1ull << id;
                         cnt += (lane == id) ? 1 : 0;
+                    };
+                    /* five draws at a time bring the generator's five words
+                     * back to their registers (no moves) */
+                    int t = 1;
+                    for (; t + 4 <= a.S; t += 5) {
+                        draw1();
+                        draw1();
+                        draw1();
+                        draw1();
+                        draw1();
                     }
+                    for (; t <= a.S; t++) draw1();
                 }
             } else {
 #pragma unroll
@@ -1841,10 +1864,10 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
          * When the current depth won, the costs are unchanged: without a
          * sentinel (which draws again) the message is the one computed at
          * the top of the row, bit for bit. */
-        if (idx == 1 && !__ballot(is_view && cst == kSentinel)) {
+        if (idx == 1 && !sentinel_lanes(cst, vmask_v)) {
             m = mf;
         } else {
-            m = g_prev + emission_lr(sentinel_emission(cnew, is_view, lane, rng), mc);
+            m = g_prev + emission_lr(sentinel_emission(cnew, is_view, lane, rng, vmask_v), mc);
         }
         d_former = best;
 

@@ -41,18 +41,20 @@ VTOL = 2e-5
 CTOL = 2e-5
 
 
-# Below this largest posterior numerator the oracle's view CDF comes from
-# FP64 underflow (a NaN CDF when every numerator is 0) or from denormal
-# numerators; the engine decides that regime in the log domain, so the two
-# need not draw the same view there (it arises only at the left/top border
-# columns, whose tile-index quirk gives costs far outside [0, 2]).
-POST_TINY = 1e-300
+# With the log of the largest posterior numerator in this band the oracle's
+# view CDF comes from denormal FP64 numerators or from their underflow to 0
+# (a NaN CDF: the last view drawn); the engine decides that regime in the
+# log domain, so the two need not draw the same view there (it arises only
+# at the left/top border columns, whose tile-index quirk gives costs far
+# outside [0, 2]).  Below the band both underflow, above it both are exact.
+LPOST_BAND = (-747.0, -707.0)
 
 
 def fragile_mask(trace, vtol=VTOL, ctol=CTOL):
     m = (trace["view_margin"] < vtol) | (trace["cost_margin"] < ctol) | (trace["near_clamp"] > 0)
-    if "post_max" in trace:
-        m |= trace["post_max"] < POST_TINY
+    if "lpost_max" in trace:
+        lp = trace["lpost_max"]
+        m |= (lp > LPOST_BAND[0]) & (lp < LPOST_BAND[1])
     return m
 
 
