@@ -618,7 +618,7 @@ static void engine_free(pmx_engine* e) {
                     e->stats,   e->Hm,      e->depth_rm, e->depth_cm, e->cost_rm, e->cost_cm,
                     e->spm,     e->rng,     e->used_ids, e->out_depth, e->out_cost, e->counters,
                     e->prop_rm, e->prop_cm, e->spread_rm, e->spread_cm, e->src16,
-                    e->src2};
+                    e->src2,    e->prop_x,  e->prop_d};
     for (void* b : bufs) dev_free(b);
     for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
     delete e;
@@ -740,7 +740,6 @@ int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format, 
     }
     ALLOC(e->src_raw, src_bytes);
     ALLOC(e->src, src_bytes);
-    ALLOC(e->src2, 2 * src_bytes); /* the default storage's texel pairs (< 4 GiB) */
     HIPCHK_E(hipMemsetAsync(e->src_raw, 0, src_bytes, s));
     e->ref_pitch = round_up(W, 64);
     const size_t plane = sizeof(float) * (size_t)e->ref_pitch * H;
@@ -777,48 +776,6 @@ int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format, 
         HIPCHK_E(er);
     }
 
-    /* homographies: FP64 on the device */
-    {
-        ALLOC(e->Hm, sizeof(double) * 18 * V);
-        std::vector<double> cams((size_t)(V + 1) * 21);
-        auto pack = [&](const pmx_camera& c, double* o) {
-            std::memcpy(o, c.K, 9 * sizeof(double));
-            std::memcpy(o + 9, c.R, 9 * sizeof(double));
-            std::memcpy(o + 18, c.C, 3 * sizeof(double));
-        };
-        pack(*ref_cam, cams.data());
-        for (int v = 0; v < V; v++) pack(src_cams[v], cams.data() + (size_t)(v + 1) * 21);
-        double* dc = nullptr;
-        ALLOC(dc, cams.size() * sizeof(double));
-        hipError_t er = hipMemcpyAsync(dc, cams.data(), cams.size() * sizeof(double),
-                                       hipMemcpyHostToDevice, s);
-        if (er == hipSuccess)
-            er = pmx_launch_compose(s, dc, V, p.orientation_x, p.orientation_z, e->Hm);
-        e->Hhost.resize((size_t)V * 18);
-        if (er == hipSuccess)
-            er = hipMemcpyAsync(e->Hhost.data(), e->Hm, sizeof(double) * 18 * V,
-                                hipMemcpyDeviceToHost, s);
-        if (er == hipSuccess) er = hipStreamSynchronize(s);
-        dev_free(dc);
-        HIPCHK_E(er);
-        /* K^-1 entries (0,0) (0,2) (1,1) (1,2) of the reference
-         * (src/patchMatch.cu:154-157), FP64 */
-        const double* K = ref_cam->K;
-        const double a = K[0], b = K[1], c = K[2], d = K[3], ee = K[4], f = K[5], g = K[6],
-                     h = K[7], i = K[8];
-        const double A = ee * i - f * h, B = -(d * i - f * g), C = d * h - ee * g;
-        const double det = a * A + b * B + c * C;
-        if (!(std::fabs(det) > 0.0)) {
-            engine_free(e);
-            return fail(ctx, PMX_ERR_ARG, "reference K is singular");
-        }
-        const double r = 1.0 / det;
-        e->invK[0] = A * r;
-        e->invK[1] = (b * f - c * ee) * r;
-        e->invK[2] = (a * i - c * g) * r;
-        e->invK[3] = -(a * f - c * d) * r;
-    }
-
     /* state */
     e->depth_rm_ld = round_up(W, 64);
     e->depth_cm_ld = round_up(H, 64);
@@ -842,6 +799,49 @@ int pmx_engine_create(pmx_ctx* ctx, const pmx_params* params, int image_format, 
     ALLOC(e->out_cost, (size_t)e->out_cost_pitch * H * V);
     ALLOC(e->counters, sizeof(unsigned long long) * 4);
     HIPCHK_E(hipMemsetAsync(e->counters, 0, sizeof(unsigned long long) * 4, s));
+
+    /* homographies: FP64 on the device */
+    {
+        ALLOC(e->Hm, sizeof(double) * 18 * V);
+        std::vector<double> cams((size_t)(V + 1) * 21);
+        auto pack = [&](const pmx_camera& c, double* o) {
+            std::memcpy(o, c.K, 9 * sizeof(double));
+            std::memcpy(o + 9, c.R, 9 * sizeof(double));
+            std::memcpy(o + 18, c.C, 3 * sizeof(double));
+        };
+        pack(*ref_cam, cams.data());
+        for (int v = 0; v < V; v++) pack(src_cams[v], cams.data() + (size_t)(v + 1) * 21);
+        /* the cameras pass through the message plane, not yet in use (no
+         * temporary allocation: a device free waits for the whole device) */
+        double* dc = (double*)e->spm;
+        hipError_t er = hipMemcpyAsync(dc, cams.data(), cams.size() * sizeof(double),
+                                       hipMemcpyHostToDevice, s);
+        if (er == hipSuccess)
+            er = pmx_launch_compose(s, dc, V, p.orientation_x, p.orientation_z, e->Hm);
+        e->Hhost.resize((size_t)V * 18);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(e->Hhost.data(), e->Hm, sizeof(double) * 18 * V,
+                                hipMemcpyDeviceToHost, s);
+        if (er == hipSuccess) er = hipStreamSynchronize(s);
+        HIPCHK_E(er);
+        /* K^-1 entries (0,0) (0,2) (1,1) (1,2) of the reference
+         * (src/patchMatch.cu:154-157), FP64 */
+        const double* K = ref_cam->K;
+        const double a = K[0], b = K[1], c = K[2], d = K[3], ee = K[4], f = K[5], g = K[6],
+                     h = K[7], i = K[8];
+        const double A = ee * i - f * h, B = -(d * i - f * g), C = d * h - ee * g;
+        const double det = a * A + b * B + c * C;
+        if (!(std::fabs(det) > 0.0)) {
+            engine_free(e);
+            return fail(ctx, PMX_ERR_ARG, "reference K is singular");
+        }
+        const double r = 1.0 / det;
+        e->invK[0] = A * r;
+        e->invK[1] = (b * f - c * ee) * r;
+        e->invK[2] = (a * i - c * g) * r;
+        e->invK[3] = -(a * f - c * d) * r;
+    }
+
     HIPCHK_E(hipStreamSynchronize(s));
 #undef ALLOC
 #undef HIPCHK_E
@@ -888,6 +888,9 @@ int pmx_engine_footprint(const pmx_params* params, int ref_width, int ref_height
     if (with_proposal) {
         t += 2 * g(sizeof(float) * (size_t)round_up(ref_width, 64) * H);
         t += 2 * g(sizeof(float) * (size_t)round_up(ref_height, 64) * W);
+        /* pmx_engine_propose's network input and proposal planes */
+        t += g(sizeof(uint16_t) * (size_t)((ref_width + 3) / 4) * ((ref_height + 3) / 4) * 64);
+        t += g(2 * sizeof(float) * W * H);
     }
     *bytes = t;
     return PMX_OK;
@@ -930,7 +933,10 @@ static int prepare_impl(pmx_engine* e, Timer* tm) {
         HIPCHK(ctx, pmx_launch_quantize_f16(s, e->ref, nullptr, (size_t)e->ref_pitch * H));
     } else {
         /* FP32 storage: each texel with its right neighbour's difference,
-         * so a bilinear row blend is one FMA */
+         * so a bilinear row blend is one FMA (the pair block, under 4 GiB,
+         * is allocated at the first FP32 prepare: engines switched to FP16
+         * before preparing never hold it) */
+        if (!e->src2) PMXCHK(dev_alloc(ctx, (void**)&e->src2, 2 * src_bytes));
         HIPCHK(ctx, pmx_launch_pair_f32(s, e->src, e->src2, (size_t)e->src_layer * V));
     }
     /* reference box sums and NCC statistics (FP64) */
@@ -1029,8 +1035,6 @@ int pmx_engine_set_storage(pmx_engine* e, int storage) {
     if (storage == PMX_STORE_F16 && !e->src16)
         PMXCHK(dev_alloc(e->ctx, (void**)&e->src16,
                          sizeof(uint16_t) * (size_t)e->src_layer * e->V));
-    if (storage == PMX_STORE_F32 && !e->src2)
-        PMXCHK(dev_alloc(e->ctx, (void**)&e->src2, sizeof(float2) * (size_t)e->src_layer * e->V));
     if (storage == PMX_STORE_F16 && e->src2) {
         /* the FP32 pairs are rebuilt if FP32 storage is chosen again */
         HIPCHK(e->ctx, hipStreamSynchronize(e->ctx->stream));
@@ -1136,19 +1140,19 @@ int pmx_engine_propose(pmx_engine* e, pmx_net* net, int view0, int view1, float 
     pmx_ctx* ctx = e->ctx;
     const int W0 = (e->W + 3) / 4, H0 = (e->H + 3) / 4;
     if (view0 < 0 || view1 < 0) volume_views(e, &view0, &view1);
-    void* x = nullptr;
-    float* d = nullptr;
-    PMXCHK(dev_alloc(ctx, &x, sizeof(uint16_t) * (size_t)W0 * H0 * 64));
-    int rc = dev_alloc(ctx, (void**)&d, 2 * sizeof(float) * (size_t)e->W * e->H);
+    if (!e->prop_x) PMXCHK(dev_alloc(ctx, &e->prop_x, sizeof(uint16_t) * (size_t)W0 * H0 * 64));
+    if (!e->prop_d)
+        PMXCHK(dev_alloc(ctx, (void**)&e->prop_d, 2 * sizeof(float) * (size_t)e->W * e->H));
+    void* x = e->prop_x;
+    float* d = e->prop_d;
     float* sp = d + (size_t)e->W * e->H;
+    int rc = PMX_OK;
     if (rc == PMX_OK) rc = pmx_engine_cost_volume(e, view0, view1, x);
     if (rc == PMX_OK) rc = pmx_net_forward(net, x, 1, H0, W0);
     if (rc == PMX_OK)
         rc = pmx_net_propose(net, e->W, e->H, tau, d, e->W * 4, sp, e->W * 4, nullptr, 0);
     if (rc == PMX_OK) rc = pmx_engine_set_proposal(e, d, sp, e->W * 4);
     hipError_t es = hipStreamSynchronize(ctx->stream);
-    dev_free(x);
-    dev_free(d);
     if (rc == PMX_OK && es != hipSuccess)
         rc = fail(ctx, PMX_ERR_HIP, "%s", hipGetErrorString(es));
     return rc;

@@ -144,6 +144,11 @@ struct pmx_engine {
     float* prop_cm = nullptr;   int prop_cm_ld = 0;
     float* spread_rm = nullptr; /* proposal spread, same layouts and lds */
     float* spread_cm = nullptr;
+    /* pmx_engine_propose's working buffers (network input, proposal planes),
+     * kept from its first call to the engine's end: a device free waits for
+     * the whole device, which would stall a batch running on another stream */
+    void* prop_x = nullptr;
+    float* prop_d = nullptr;
     bool has_prop = false;
     bool prop_seeds = true;     /* the proposal seeds the depth */
     int prop_sweeps = 0;        /* sweeps of a run that get the proposal (0: all) */

@@ -37,9 +37,12 @@ def test_footprint_closed_form():
     # 8-byte FP32 texel pairs
     g = lambda b: (b + 255) // 256 * 256  # noqa: E731
     assert f32 - f16 == g(8 * layer * V) - g(2 * layer * V)
-    # proposal planes: depth and spread, row- and column-major
+    # proposal planes: depth and spread, row- and column-major; the
+    # proposal's network input (64 FP16 channels at quarter resolution) and
+    # its two full-resolution output planes
     ld_r, ld_c = (W + 63) // 64 * 64, (H + 63) // 64 * 64
-    assert fp - f16 == 2 * ((4 * ld_r * H + 255) // 256 * 256) + 2 * ((4 * ld_c * W + 255) // 256 * 256)
+    W0, H0 = (W + 3) // 4, (H + 3) // 4
+    assert fp - f16 == 2 * g(4 * ld_r * H) + 2 * g(4 * ld_c * W) + g(2 * W0 * H0 * 64) + g(8 * W * H)
     # dominant terms: two FP32 source blocks and the texel pairs, the
     # messages and the output costs, two cost layouts
     dominant = 2 * 4 * layer * V + 8 * layer * V + 8 * W * H * V + 2 * 4 * W * H * V
