@@ -134,6 +134,13 @@ __device__ __forceinline__ gptr<T> vglobal(T* p) {
     asm volatile("" : "+v"(g));
     return g;
 }
+/* ... or left where the compiler puts it (scalar registers), for kernels
+ * whose vector registers are the scarcer ones */
+template <bool VEC, class T>
+__device__ __forceinline__ gptr<T> col_ptr(T* p) {
+    if constexpr (VEC) return vglobal(p);
+    return (gptr<T>)p;
+}
 
 __device__ __forceinline__ float uniform_f(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
@@ -1414,14 +1421,18 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
 
     /* this column's depth, costs, backward messages, proposal and window
      * statistics (x = c in a plain sweep, y = c in a rotated one) */
-    const gptr<float> dcol = vglobal(a.depth + (size_t)c * a.depth_ld);
-    const gptr<float> ccol = vglobal(a.cost + (size_t)c * a.cost_ld);
-    const gptr<float> scol = vglobal(a.spm + (size_t)c * a.spm_ld);
-    const gptr<const float> pcol = vglobal(prp ? a.prop + (size_t)c * a.prop_ld : a.depth);
-    const gptr<const float> qcol = vglobal(prp ? a.prop_spread + (size_t)c * a.prop_ld : a.depth);
+    /* Kernels with a third resident wave per SIMD (NB > 0: tap batches cut
+     * to fit 168 registers) keep the column pointers in scalar registers
+     * and project phase C's hypotheses again instead of holding phase B's */
+    constexpr bool kVec = NB == 0;
+    const gptr<float> dcol = col_ptr<kVec>(a.depth + (size_t)c * a.depth_ld);
+    const gptr<float> ccol = col_ptr<kVec>(a.cost + (size_t)c * a.cost_ld);
+    const gptr<float> scol = col_ptr<kVec>(a.spm + (size_t)c * a.spm_ld);
+    const gptr<const float> pcol = col_ptr<kVec>(prp ? a.prop + (size_t)c * a.prop_ld : a.depth);
+    const gptr<const float> qcol = col_ptr<kVec>(prp ? a.prop_spread + (size_t)c * a.prop_ld : a.depth);
     typedef float f32x2 __attribute__((ext_vector_type(2)));
     const gptr<const f32x2> stcol =
-        vglobal((const f32x2*)(a.stats + (rot ? (size_t)c * a.stats_pitch : (size_t)c)));
+        col_ptr<kVec>((const f32x2*)(a.stats + (rot ? (size_t)c * a.stats_pitch : (size_t)c)));
     const int ststep = rot ? 1 : a.stats_pitch;
 
     /* generator of this column: row 0 (plain) or column 0 (rotated) */
@@ -1507,8 +1518,10 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
     if (Hs_rows > 1) D = store_window<PF, kWinAll>(winbuf, pf_win, pf_stat.x, Wn, lane, need_d);
 
     unsigned long long n_evals = 0, n_samples = 0;
-    to_vgpr(n_evals);
-    to_vgpr(n_samples);
+    if constexpr (kVec) {
+        to_vgpr(n_evals);
+        to_vgpr(n_samples);
+    }
     int pb = 0; /* pass counter: parity selects the partial-sum buffer */
 
     /* one phase of hypotheses tview/tdep[0..nt) -> tout[0..nt); in the
@@ -1838,6 +1851,7 @@ __device__ __forceinline__ void sweep_column(const PmxSweepArgs& a, const int c)
                 const int kc = idx == 0 ? 0 : (idx == 2 ? 1 : 2);
                 const bool on = lead && hk == kc && hv < V && !((sel >> hv) & 1ull);
                 const int rk = __popc((uint32_t)rest & ((1u << hv) - 1u));
+                if constexpr (!kVec) hq = make_hyp(Hs, sb, hv < V ? hv : 0, best, pc);
                 store_hyp(on ? hypb + 4 * rk : (float4*)tview + lane, hq, on ? 1 : 0);
                 lds_barrier();
             } else {
