@@ -378,7 +378,9 @@ int pmx_net_propose(pmx_net* net, int width, int height, float tau, float* depth
                     int depth_pitch_bytes, float* spread_dev, int spread_pitch_bytes,
                     float* conf_dev, int conf_pitch_bytes);
 /* Cost volume -> forward -> proposal -> pmx_engine_set_proposal.  view0 or
- * view1 < 0 selects pmx_engine_volume_views. */
+ * view1 < 0 selects pmx_engine_volume_views.  Its working buffers (network
+ * input, proposal planes) are allocated at the first call and kept until
+ * pmx_engine_destroy (no device free while other streams run). */
 int pmx_engine_propose(pmx_engine* eng, pmx_net* net, int view0, int view1, float tau);
 
 /* Single operators of the network, on device bf16 NHWC tensors (channel
