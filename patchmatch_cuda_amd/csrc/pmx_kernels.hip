@@ -851,15 +851,16 @@ __device__ __forceinline__ float ncc_finish_sel(float A, float B, float C, float
  * 1 - NCC is written to tout[t]; otherwise each wave writes its partial
  * sums to part[t * NW + slice] and the caller combines them after a
  * barrier (combine_pass).  PRE: the caller passes each lane group's
- * projection in qpre and every group's sums go to part (see below).
- * Returns k.
+ * projection in qpre and every group's sums go to part (see below), the
+ * other lanes' stores to part[spare + lane].  Returns k.
  */
 template <int G, int WN, int NW, int NB = 0, bool INT = false, bool PRE = false, class SB>
 __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __restrict__ Hm,
                                          const SB& sb, const float* __restrict__ win,
                                          int wave, int lane, int nt, const int* tview,
                                          const float* tdep, float* tout, float4* part,
-                                         const PixelCtx& pc, const Hyp* qpre = nullptr) {
+                                         const PixelCtx& pc, const Hyp* qpre = nullptr,
+                                         int spare = 40 * NW) {
     constexpr int NG = kLanes / G;
     constexpr int NGT = NG * NW;
     /* tap rows in flight per lane: a whole column with one wave per
@@ -942,10 +943,10 @@ __device__ __forceinline__ int eval_pass(const PmxSweepArgs& a, const double* __
     }
     if constexpr (PRE) {
         /* every lane stores: the group's writer its sums, the others to a
-         * spare slot of their own, part[40 NW + lane], past the at most
-         * 24 NW slots of a phase's hypotheses (no branch) */
+         * spare slot of their own, part[spare + lane] (by default past the
+         * at most 24 NW slots of a phase's hypotheses; no branch) */
         const bool wr = active && (k <= NG ? sub == last : (sub & (NG - 1)) == last) && gl == 0;
-        part[wr ? t * NW + (k <= NG ? 0 : sub / NG) : 40 * NW + lane] = make_float4(A, B, C, K);
+        part[wr ? t * NW + (k <= NG ? 0 : sub / NG) : spare + lane] = make_float4(A, B, C, K);
     } else if (k <= NG) {
         if (active && sub == last && gl == 0) tout[t] = ncc_finish(A, B, C, K, pc);
     } else if (active && (sub & (NG - 1)) == last && gl == 0) {
@@ -1148,10 +1149,9 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
     extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
     double* Hs = smem_d;                               /* V x 18 */
-    float* win = (float*)(smem_d + a.V * 18);          /* Wn rows + Wn zero rows, x Wn */
-    int* tview = (int*)(win + 2 * Wn * Wn);            /* 64 */
-    float* tdep = (float*)(tview + 64);                /* 64 */
-    float* tout = tdep + 64;                           /* 64 */
+    float4* part = (float4*)(smem_d + a.V * 18);       /* 128: sums per view, then spare */
+    float4* hypb = part + 128;                         /* V projections, 4 float4 each */
+    float* win = (float*)(hypb + 4 * a.V);             /* Wn rows + Wn zero rows, x Wn */
     const int lane = threadIdx.x;
     const int gl = lane % G;
     constexpr int NG = kLanes / G;
@@ -1173,17 +1173,25 @@ __global__ __launch_bounds__(64) void pmx_cost_all_kernel(PmxSweepArgs a) {
     pc.D = store_window<PF>(win, vals, pc.mu, Wn, lane, qx != 0 || qy != 0);
     /* state is row-major: column c = y, row r = x */
     const float depth = a.depth[(size_t)y * a.depth_ld + x];
-    if (lane < a.V) {
-        tview[lane] = lane;
-        tdep[lane] = depth;
+    /* each view's projection once, by lane v; the passes leave their window
+     * sums in part[v] (spare stores at part[64 + lane]); lane v forms view
+     * v's 1 - NCC */
+    if (lane < a.V) store_hyp(hypb + 4 * lane, make_hyp(Hs, sb, lane, depth, pc));
+    __syncthreads();
+    const int gg = lane / G;
+    for (int v0 = 0; v0 < a.V; v0 += NG) {
+        const int ntp = min(NG, a.V - v0);
+        const int lk = __builtin_ctz((unsigned)NG) - (ntp > 1 ? 32 - __builtin_clz((unsigned)(ntp - 1)) : 0);
+        const Hyp q = load_hyp(hypb + 4 * (v0 + min(gg >> lk, ntp - 1)));
+        eval_pass<G, WN, 1, 0, false, true>(a, Hs, sb, win, 0, lane, ntp, nullptr, nullptr, nullptr,
+                                            part + v0, pc, &q, 64 - v0);
     }
     __syncthreads();
-    for (int v0 = 0; v0 < a.V; v0 += NG)
-        eval_pass<G, WN, 1>(a, Hs, sb, win, 0, lane, min(NG, a.V - v0), tview + v0, tdep + v0,
-                            tout + v0, nullptr, pc);
-    __syncthreads();
     float* cst = a.cost + (size_t)y * a.cost_ld + (size_t)x * a.V;
-    if (lane < a.V) cst[lane] = tout[lane];
+    if (lane < a.V) {
+        const float4 s4 = part[lane];
+        cst[lane] = ncc_finish(s4.x, s4.y, s4.z, s4.w, pc);
+    }
     /* the evaluation counts of this kernel are a closed form of the image
      * size (counted on the host): no per-pixel atomics on shared counters */
 }
@@ -2062,7 +2070,8 @@ namespace {
 template <int G, int PF, int WN>
 hipError_t launch_cost_all_t(hipStream_t s, const PmxSweepArgs& a) {
     const int Wn = 2 * a.hw + 1;
-    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float) * (2 * Wn * Wn + 3 * 64);
+    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * (128 + 4 * a.V) +
+                      sizeof(float) * (2 * Wn * Wn);
     const long tiles = (long)((a.W + kCostTW - 1) / kCostTW) * ((a.H + kCostTH - 1) / kCostTH);
     const dim3 grid((unsigned)(tiles * kCostTW * kCostTH));
     if (a.src_h16)
