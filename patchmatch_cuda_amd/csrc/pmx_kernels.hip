@@ -1208,10 +1208,10 @@ __global__ __launch_bounds__(64) void pmx_volume_kernel(PmxSweepArgs a, int view
     extern __shared__ double smem_d[];
     const int Wn = WN > 0 ? WN : 2 * a.hw + 1;
     double* Hs = smem_d;                               /* V x 18 */
-    float* win = (float*)(smem_d + a.V * 18);          /* Wn rows + Wn zero rows, x Wn */
-    int* tview = (int*)(win + 2 * Wn * Wn);            /* 64 */
-    float* tdep = (float*)(tview + 64);                /* 64 */
-    float* tout = tdep + 64;                           /* 64 */
+    float4* part = (float4*)(smem_d + a.V * 18);       /* 128: sums per hypothesis, then spare */
+    float4* hypb = part + 128;                         /* 64 projections, 4 float4 each */
+    float* tout = (float*)(hypb + 4 * 64);             /* 64 */
+    float* win = tout + 64;                            /* Wn rows + Wn zero rows, x Wn */
     const int lane = threadIdx.x;
     const int gl = lane % G;
     constexpr int NG = kLanes / G;
@@ -1232,15 +1232,25 @@ __global__ __launch_bounds__(64) void pmx_volume_kernel(PmxSweepArgs a, int view
     load_window<PF>(a, x, y, qx, 0, Wn, inv_wn, lane, vals);
     pc.D = store_window<PF>(win, vals, pc.mu, Wn, lane, qx != 0);
     {
+        /* hypothesis `lane`: plane lane & 31 on view0 (lanes 0-31) or view1,
+         * projected once by its lane; the passes leave the window sums in
+         * part, each lane forms its hypothesis' 1 - NCC */
         const int k = lane & 31;
         const float ulo = 1.0f / a.far_d, uhi = 1.0f / a.near_d;
-        tview[lane] = lane < 32 ? view0 : view1;
-        tdep[lane] = 1.0f / (ulo + (uhi - ulo) * ((float)k + 0.5f) * (1.0f / 32.0f));
+        const float dk = 1.0f / (ulo + (uhi - ulo) * ((float)k + 0.5f) * (1.0f / 32.0f));
+        store_hyp(hypb + 4 * lane, make_hyp(Hs, sb, lane < 32 ? view0 : view1, dk, pc));
     }
     __syncthreads();
-    for (int t0 = 0; t0 < 64; t0 += NG)
-        eval_pass<G, WN, 1>(a, Hs, sb, win, 0, lane, NG, tview + t0, tdep + t0, tout + t0, nullptr,
-                            pc);
+    for (int t0 = 0; t0 < 64; t0 += NG) {
+        const Hyp q = load_hyp(hypb + 4 * (t0 + lane / G));
+        eval_pass<G, WN, 1, 0, false, true>(a, Hs, sb, win, 0, lane, NG, nullptr, nullptr, nullptr,
+                                            part + t0, pc, &q, 64 - t0);
+    }
+    __syncthreads();
+    {
+        const float4 s4 = part[lane];
+        tout[lane] = ncc_finish(s4.x, s4.y, s4.z, s4.w, pc);
+    }
     __syncthreads();
     float v = 0.0f;
     if (lane < 32)
@@ -2085,7 +2095,8 @@ template <int G, int PF, int WN>
 hipError_t launch_volume_t(hipStream_t s, const PmxSweepArgs& a, int v0, int v1, uint16_t* out,
                            int W0, int H0) {
     const int Wn = 2 * a.hw + 1;
-    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float) * (2 * Wn * Wn + 3 * 64);
+    const size_t sh = sizeof(double) * a.V * 18 + sizeof(float4) * (128 + 4 * 64) +
+                      sizeof(float) * (64 + 2 * Wn * Wn);
     const dim3 grid((unsigned)((long)W0 * H0));
     if (a.src_h16)
         hipLaunchKernelGGL((pmx_volume_kernel<G, PF, WN, true>), grid, dim3(64), sh, s, a, v0, v1,
