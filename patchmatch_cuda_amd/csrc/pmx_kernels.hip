@@ -607,10 +607,12 @@ template <class SB>
 __device__ __forceinline__ Hyp make_hyp(const double* __restrict__ Hm, const SB& sb, int view,
                                         float depth, const PixelCtx& pc) {
     const double* __restrict__ Hv = Hm + view * 18;
-    const double inv = rcp64((double)depth * pc.scale);
+    /* the plane homography H1 - H2 / (depth scale), scaled by depth scale
+     * (the projection is a ratio: no reciprocal needed) */
+    const double sd = (double)depth * pc.scale;
     double h[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) h[k] = fma(-Hv[9 + k], inv, Hv[k]);
+    for (int k = 0; k < 9; k++) h[k] = fma(Hv[k], sd, -Hv[9 + k]);
     const double X0 = fma(h[0], pc.cx, fma(h[1], pc.cy, h[2]));
     const double Y0 = fma(h[3], pc.cx, fma(h[4], pc.cy, h[5]));
     const double Z0 = fma(h[6], pc.cx, fma(h[7], pc.cy, h[8]));
