@@ -76,6 +76,34 @@ def test_storage_switch_and_rounding(gpu_ctx):
     eng.close()
 
 
+def test_storage_round_trip_runs(gpu_ctx):
+    """FP32 storage (the texel-pair block, allocated at the first FP32
+    prepare), then FP16 (which frees it), then FP32 again (a new pair block):
+    the two FP32 runs give the same depths, costs and generator states, bit
+    for bit, and the FP16 run differs from them."""
+    spec, rp = configs.C1_SMALL
+    _say("case: storage FP32 -> FP16 -> FP32, full runs (C1)")
+    sc, eng = make_engine(gpu_ctx, spec, rp, prepare=False)
+    state = lambda: (eng.depth(), eng.costs(), eng.rng().view(np.uint32))  # noqa: E731
+    try:
+        eng.prepare()
+        eng.run()
+        a = state()
+        eng.set_storage(capi.STORE_F16)
+        eng.prepare()
+        eng.run()
+        h = state()
+        eng.set_storage(capi.STORE_F32)
+        eng.prepare()
+        eng.run()
+        b = state()
+        for name, x, y in zip(("depth", "costs", "rng"), a, b):
+            assert np.array_equal(x, y), f"second FP32 run: {name} differs"
+        assert not np.array_equal(a[0], h[0])
+    finally:
+        eng.close()
+
+
 def test_fp16_cost_all_views_within_1e5(c1_16):
     spec, rp, sc, eng = c1_16
     _say("case: fp16 cost all views c1")
