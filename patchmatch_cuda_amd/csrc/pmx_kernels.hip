@@ -270,7 +270,10 @@ __global__ __launch_bounds__(256) void transpose_f32_kernel(const float* __restr
 }
 
 /* Transpose of E-float records: in[y][x][e] -> out[x][y][e]; T x T records
- * per workgroup (T = 16, or 8 for E > 32 to bound LDS). */
+ * per workgroup (T = 16, or 8 for E > 32 to bound LDS).  Thread t moves the
+ * floats f = t + 256 j of every tile row (T E <= 512, so j < 2): the
+ * divisions by E that locate them in the transposed tile are done once per
+ * thread, not per float. */
 __global__ __launch_bounds__(256) void transpose_rec_kernel(const float* __restrict__ in,
                                                             float* __restrict__ out, int width,
                                                             int height, int E, int T, int ld_in,
@@ -280,18 +283,23 @@ __global__ __launch_bounds__(256) void transpose_rec_kernel(const float* __restr
     const int bx = blockIdx.x * T, by = blockIdx.y * T;
     const int t = threadIdx.x;
     const int nx = min(T, width - bx), ny = min(T, height - by);
-    const int span_in = nx * E;
-    for (int i = t; i < T * T * E; i += 256) {
-        const int r = i / (T * E), f = i % (T * E);
-        if (r < ny && f < span_in) tile[r * row_len + f] = in[(size_t)(by + r) * ld_in + (size_t)bx * E + f];
+    const int span_in = nx * E, span_out = ny * E;
+    const float* __restrict__ src = in + (size_t)by * ld_in + (size_t)bx * E;
+    float* __restrict__ dst = out + (size_t)bx * ld_out + (size_t)by * E;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int f = t + 256 * j;
+        if (f < span_in)
+            for (int r = 0; r < ny; r++) tile[r * row_len + f] = src[(size_t)r * ld_in + f];
     }
     __syncthreads();
-    const int span_out = ny * E;
-    for (int i = t; i < T * T * E; i += 256) {
-        const int xr = i / (T * E), f = i % (T * E);
-        if (xr < nx && f < span_out) {
-            const int yy = f / E, e = f % E;
-            out[(size_t)(bx + xr) * ld_out + (size_t)by * E + f] = tile[yy * row_len + xr * E + e];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int f = t + 256 * j;
+        if (f < span_out) {
+            const int yy = f / E, e = f - yy * E;
+            const float* __restrict__ tp = tile + yy * row_len + e;
+            for (int xr = 0; xr < nx; xr++) dst[(size_t)xr * ld_out + f] = tp[xr * E];
         }
     }
 }
@@ -2000,6 +2008,7 @@ hipError_t pmx_launch_transpose_rec(hipStream_t s, const float* in, float* out, 
                                     int height, int E, int ld_in, int ld_out) {
     if (E == 1) return pmx_launch_transpose_f32(s, in, out, width, height, ld_in, ld_out);
     const int T = E > 32 ? 8 : 16;
+    if (E < 1 || T * E > 512) return hipErrorInvalidValue; /* one tile row per <= 2 floats a thread */
     dim3 b(256), g((width + T - 1) / T, (height + T - 1) / T);
     const size_t sh = sizeof(float) * T * (T * E + 1);
     hipLaunchKernelGGL(transpose_rec_kernel, g, b, sh, s, in, out, width, height, E, T, ld_in,
