@@ -568,12 +568,12 @@ __device__ __forceinline__ TapW tap_fetch(const SB& sb, const Hyp& q, float sx, 
         t0 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, 0, 0));
         t1 = __builtin_bit_cast(float2,
                                 __builtin_amdgcn_raw_buffer_load_b64(sb.rsrc, off, sb.pitch_bytes, 0));
-        /* off stays live past the second load, so the register allocator
-         * does not give that load off's register as its destination: a load
-         * overwriting the address of the load before it costs a wait state
-         * (s_nop) between the two on gfx950, one per tap */
-        asm volatile("" ::"v"(off));
     }
+    /* off stays live past the second load, so the register allocator does
+     * not give that load off's register as its destination: a load
+     * overwriting the address of the load before it costs a wait state
+     * (s_nop) between the two on gfx950, one per tap */
+    asm volatile("" ::"v"(off));
     return TapW{__builtin_amdgcn_fractf(sx), __builtin_amdgcn_fractf(sy)};
 }
 
