@@ -8,7 +8,12 @@ the engine's own images (teacher forcing):
 * sources of different sizes than the reference and each other (packed and
   zero-padded like copyData, src/patchMatch.cu:51-130);
 * a single source view (the minimum, src/patchMatch.cu:138-142) and an
-  image whose sides are not multiples of any block size.
+  image whose sides are not multiples of any block size;
+* more than 16 source views or more than 7 draws per row (the general
+  sweep kernels: the view CDF over the whole wave, draws resolved by one
+  compare of floor(64 / V) draws at a time) and, above 32 views, the cost
+  record transposes in 8 x 8 record tiles (src/patchMatch.cu:888-961 for the
+  per-view messages and draws, which take any view count).
 
 Bars as for the main parity tests: costs within 1e-5, a teacher-forced
 sweep decision for decision, >= 99% of final depths within 1%."""
@@ -98,4 +103,14 @@ def test_single_source_and_odd_size(gpu_ctx):
     sc = scene.render(spec)
     eng = capi.Engine(gpu_ctx, rp, sc.ref, list(sc.srcs), sc.cams)
     _check(gpu_ctx, sc, eng, rp, "V=1 33x17")
+    eng.close()
+
+
+@pytest.mark.parametrize("V,S,hw", [(5, 12, 2), (20, 9, 3), (40, 3, 7)])
+def test_many_views_and_draws(gpu_ctx, V, S, hw):
+    _say(f"case: {V} sources, S = {S}, window {2 * hw + 1}x{2 * hw + 1} on 48x40")
+    spec, rp = configs.tiny(W=48, H=40, V=V, hw=hw, S=S, iters=2, seed=61 + V)
+    sc = scene.render(spec)
+    eng = capi.Engine(gpu_ctx, rp, sc.ref, list(sc.srcs), sc.cams)
+    _check(gpu_ctx, sc, eng, rp, f"V={V} S={S}")
     eng.close()
