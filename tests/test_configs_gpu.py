@@ -89,12 +89,12 @@ def test_c2_costs_within_1e5(c2_crop):
     assert diff.max() < 1e-5
 
 
-@pytest.mark.parametrize("waves", [1, 2])
+@pytest.mark.parametrize("waves", [1, 2, 4])
 @pytest.mark.parametrize("down,rotated", [(1, 0), (0, 1)])
 def test_c2_sweep_teacher_forced(c2_crop, waves, down, rotated):
     """S = 5 on ten views: up to ten phase-B hypotheses per row, spread over
     several passes of four lane groups (one wave per column) or combined
-    across two waves."""
+    across two or four waves."""
     rp, (ref, srcs, cams, gt), eng = c2_crop
     _say(f"case: 2048x1536 workload crop, sweep down={down} rotated={rotated} waves={waves}")
     eng.set_sweep_waves(waves)
