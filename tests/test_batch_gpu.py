@@ -83,6 +83,30 @@ def test_batch_equals_standalone(views, gpu_ctx, graph):
     b.close()
 
 
+def test_batch_many_views(gpu_ctx):
+    """Members with 3, 20 and 40 source views at 9 draws per row: the batch
+    runs the general sweep kernels (the view CDF over the whole wave, draws
+    resolved by one compare of floor(64 / V) draws at a time) for every
+    member, and each member equals its standalone run."""
+    _say("case: batch of views with 3, 20 and 40 sources, S = 9")
+    engs = []
+    for W, H, V, seed in [(40, 30, 3, 71), (36, 28, 20, 72), (32, 24, 40, 73)]:
+        spec, rp = configs.tiny(W=W, H=H, V=V, hw=2, S=9, iters=2, seed=seed)
+        sc, eng = make_engine(gpu_ctx, spec, rp, prepare=False)
+        engs.append((spec, rp, sc, eng))
+    try:
+        want = _standalone(engs)
+        b = capi.Batch(gpu_ctx, [e[3] for e in engs])
+        _say(f"  launch shape {b.launch()}")
+        b.prepare()
+        b.run()
+        _assert_same(engs, want, "many views")
+        b.close()
+    finally:
+        for e in engs:
+            e[3].close()
+
+
 @pytest.mark.parametrize("waves,workgroups", [(1, 8), (2, 0), (2, 24)])
 def test_batch_launch_shapes(views, gpu_ctx, waves, workgroups):
     """Few persistent workgroups (each drains many column strips) or two
